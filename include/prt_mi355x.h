@@ -1,0 +1,124 @@
+/* C ABI of the MI355X (gfx950) nearest-hit backend for portableRT.
+ *
+ * Plain pointers and sizes only. Every entry point returns a prt_status; on failure the
+ * context's last error holds a message (prt_mi355x_last_error). Nothing aborts the process:
+ * with no GPU, probe reports zero devices. A context is not thread-safe (as the reference
+ * backends, include/portableRT/backend.hpp:39-46); calls on one context block unless named
+ * *_device.
+ *
+ * Triangle layout: 9 floats v0 v1 v2 per triangle (reference Tri, core.hpp:24-25).
+ * Ray layout:      6 floats origin, direction (reference Ray, core.hpp:19-22), 24 bytes.
+ * Hit layout:      the reference HitReg<Tags...> for a field mask (hitreg.hpp:29-61).
+ */
+#ifndef PRT_MI355X_H
+#define PRT_MI355X_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct prt_ctx prt_ctx;
+
+typedef enum {
+	PRT_OK = 0,
+	PRT_ERR_ARG = 1,      /* bad argument (null pointer, size, option) */
+	PRT_ERR_NODEV = 2,    /* no gfx950 device / device index out of range */
+	PRT_ERR_HIP = 3,      /* HIP runtime error (message in last_error) */
+	PRT_ERR_OOM = 4,      /* device allocation failed or over the configured limit */
+	PRT_ERR_STATE = 5,    /* nearest_hits before set_tris */
+	PRT_ERR_LAYOUT = 6,   /* stride/offsets do not match HitReg for the field mask */
+	PRT_ERR_INPUT = 7     /* non-finite triangle vertex */
+} prt_status;
+
+/* Field mask bits: the reference filter tags (hitreg.hpp:48-54). */
+enum { PRT_UV = 1, PRT_T = 2, PRT_PRIMITIVE_ID = 4, PRT_P = 8, PRT_VALID = 16 };
+
+/* Tunables (prt_mi355x_set_option). */
+enum {
+	PRT_OPT_NODE_FORMAT = 1,  /* 0 fp32 BVH8 nodes (208 B), 1 8-bit quantized (80 B, default) */
+	PRT_OPT_CULL = 2,         /* 0 exact: no distance culling; 1 culling with margin (default) */
+	PRT_OPT_LEAF_SIZE = 3,    /* max triangles per leaf, 1..4 (default 2); takes effect at set_tris */
+	PRT_OPT_BLOCK = 4,        /* traversal workgroup size: 64, 128, 256 */
+	PRT_OPT_WAVES_PER_CU = 5, /* persistent waves per CU */
+	PRT_OPT_LDS_STACK = 6,    /* LDS stack entries per lane: 4, 8, 16 */
+	PRT_OPT_FETCH_MIN = 7,    /* idle lanes that trigger a refill (64 = refill only when all idle) */
+	PRT_OPT_STEPS = 8,        /* traversal steps between refill checks */
+	PRT_OPT_COUNT_STATS = 9,  /* 1: count node visits / triangle tests (slower) */
+	PRT_OPT_TIME_KERNEL = 10, /* 1: time the traversal kernel with HIP events (synchronises) */
+	PRT_OPT_ALLOC_LIMIT = 11, /* refuse device allocations above this many bytes (0 = none) */
+	PRT_OPT_MARGIN_REL_LOG2 = 12 /* culling margin relative part 2^-k, k in 4..30 (default 10) */
+};
+
+/* Statistics (prt_mi355x_get_stat). */
+enum {
+	PRT_STAT_BUILD_MS = 1,    /* last set_tris, device time */
+	PRT_STAT_TRACE_MS = 2,    /* last traversal kernel (needs PRT_OPT_TIME_KERNEL) */
+	PRT_STAT_NODES = 3,       /* BVH8 nodes */
+	PRT_STAT_DEPTH = 4,       /* BVH8 levels */
+	PRT_STAT_NODE_VISITS = 5, /* last trace, with PRT_OPT_COUNT_STATS */
+	PRT_STAT_TRI_TESTS = 6,
+	PRT_STAT_LEAF_VISITS = 7,
+	PRT_STAT_NODE_BYTES = 8,  /* bytes per node of the selected format */
+	PRT_STAT_API_MS = 9,      /* last host-buffer nearest_hits, wall time incl. copies */
+	PRT_STAT_N_TRIS = 10
+};
+
+/* Number of usable gfx950 devices and the name of device 0 (name may be NULL). Returns PRT_OK
+ * with *ndev = 0 when there is no GPU. Replaces the device probe in the reference HIP
+ * backend's is_available (src/intersect_hip.cpp:353-372). */
+int prt_mi355x_probe(int *ndev, char *name, size_t cap);
+
+/* Context on device `device`: one HIP stream, the BVH and reusable staging buffers. */
+int prt_mi355x_create(int device, prt_ctx **out);
+void prt_mi355x_destroy(prt_ctx *ctx);
+
+/* Message of the last failed call on ctx ("" if none). Valid until the next call. */
+const char *prt_mi355x_last_error(const prt_ctx *ctx);
+
+/* Device name ("AMD Instinct MI355X (gfx950)"). Backend::device_name (backend.hpp:23). */
+int prt_mi355x_device_name(const prt_ctx *ctx, char *buf, size_t cap);
+
+/* Builds the BVH over n triangles in host memory (copied; the previous BVH is replaced).
+ * Backend::set_tris (backend.hpp:19, intersect_cpu.cpp:12). */
+int prt_mi355x_set_tris(prt_ctx *ctx, const float *tris9, uint64_t n);
+
+/* Same with the triangles already in device memory (read during the call only). */
+int prt_mi355x_set_tris_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
+
+/* Nearest hit of n rays in host memory, records written to host memory `out` with the
+ * HitReg layout of `mask`: `stride` = sizeof(HitReg), off[8] = offsets of u v t primitive_id
+ * valid px py pz (-1 for absent fields). Blocking. The member template
+ * nearest_hits<Tags...>(const std::vector<Ray>&) (intersect_cpu.hpp:20-43,
+ * nearest_hits_impl.hpp:28-36). Miss records: t = +inf, valid = 0, u = v = 0,
+ * primitive_id = 0xFFFFFFFF, p = o + inf * d. */
+int prt_mi355x_nearest_hits(prt_ctx *ctx, const float *rays6, uint64_t n, uint32_t mask, void *out,
+                            uint32_t stride, const int32_t off[8]);
+
+/* Same with rays and records in device memory; enqueued on `stream` (hipStream_t, NULL = the
+ * context's stream) without synchronising unless PRT_OPT_TIME_KERNEL is set. No counterpart in
+ * the reference, which copies on every call (src/intersect_hip.cpp:322-339). */
+int prt_mi355x_nearest_hits_device(prt_ctx *ctx, const float *d_rays6, uint64_t n, uint32_t mask,
+                                   void *d_out, uint32_t stride, const int32_t off[8], void *stream);
+
+/* Waits for all work on the context's stream. */
+int prt_mi355x_synchronize(prt_ctx *ctx);
+
+int prt_mi355x_set_option(prt_ctx *ctx, int option, int64_t value);
+int prt_mi355x_get_option(const prt_ctx *ctx, int option, int64_t *value);
+int prt_mi355x_get_stat(const prt_ctx *ctx, int stat, double *value);
+
+/* HitReg layout for a field mask: out[0] = size, out[1..8] = offsets (u v t id valid px py pz),
+ * -1 for absent fields. Needs no device. */
+int prt_mi355x_layout(uint32_t mask, int32_t out[9]);
+
+/* Copies the device BVH to host memory for inspection: which = 0 quantized nodes, 1 fp32
+ * nodes, 2 leaf triangle records. *bytes is the size; with out == NULL only the size is
+ * returned. */
+int prt_mi355x_export(prt_ctx *ctx, int which, void *out, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,275 @@
+"""portableRT MI355X backend: Python view of the C ABI (include/prt_mi355x.h).
+
+Mirrors the reference's operator interface (include/portableRT/backend.hpp,
+nearest_hits_impl.hpp): ``available_backends()``, ``select_backend(b)``, ``b.set_tris(tris)``,
+``nearest_hits(rays, *tags)``, ``b.device_name()``, ``b.is_available()``. Hit records come back
+as numpy structured arrays with the exact byte layout of ``HitReg<Tags...>``
+(hitreg.hpp:29-61). Everything runs in the gfx950 HIP library; if it is not built the import
+of the native layer raises, there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libprt_mi355x.so")
+
+# filter tags (hitreg.hpp:48-54) as field-mask bits
+UV, T, PRIMITIVE_ID, P, VALID = 1, 2, 4, 8, 16
+ALL_TAGS = (UV, T, PRIMITIVE_ID, P, VALID)
+
+OK, ERR_ARG, ERR_NODEV, ERR_HIP, ERR_OOM, ERR_STATE, ERR_LAYOUT, ERR_INPUT = range(8)
+
+OPT = dict(node_format=1, cull=2, leaf_size=3, block=4, waves_per_cu=5, lds_stack=6, fetch_min=7,
+           steps=8, count_stats=9, time_kernel=10, alloc_limit=11, margin_rel_log2=12)
+STAT = dict(build_ms=1, trace_ms=2, nodes=3, depth=4, node_visits=5, tri_tests=6, leaf_visits=7,
+            node_bytes=8, api_ms=9, n_tris=10)
+
+_lib = None
+
+
+class PrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """The native library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run python portablert_amd/build.py")
+        L = ctypes.CDLL(LIB)
+        vp, u64, ci, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32
+        L.prt_mi355x_probe.argtypes = [ctypes.POINTER(ci), ctypes.c_char_p, ctypes.c_size_t]
+        L.prt_mi355x_create.argtypes = [ci, ctypes.POINTER(vp)]
+        L.prt_mi355x_destroy.argtypes = [vp]
+        L.prt_mi355x_destroy.restype = None
+        L.prt_mi355x_last_error.argtypes = [vp]
+        L.prt_mi355x_last_error.restype = ctypes.c_char_p
+        L.prt_mi355x_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.prt_mi355x_set_tris.argtypes = [vp, vp, u64]
+        L.prt_mi355x_set_tris_device.argtypes = [vp, vp, u64]
+        L.prt_mi355x_nearest_hits.argtypes = [vp, vp, u64, u32, vp, u32, vp]
+        L.prt_mi355x_nearest_hits_device.argtypes = [vp, vp, u64, u32, vp, u32, vp, vp]
+        L.prt_mi355x_synchronize.argtypes = [vp]
+        L.prt_mi355x_set_option.argtypes = [vp, ci, ctypes.c_int64]
+        L.prt_mi355x_get_option.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_int64)]
+        L.prt_mi355x_get_stat.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double)]
+        L.prt_mi355x_layout.argtypes = [u32, vp]
+        L.prt_mi355x_export.argtypes = [vp, ci, vp, ctypes.POINTER(u64)]
+        _lib = L
+    return _lib
+
+
+def mask_of(tags):
+    m = 0
+    for t in tags:
+        m |= int(t)
+    return m
+
+
+def layout(mask):
+    """(size, offsets u v t id valid px py pz) of HitReg for a field mask."""
+    out = np.zeros(9, np.int32)
+    if lib().prt_mi355x_layout(mask, out.ctypes.data_as(ctypes.c_void_p)):
+        raise ValueError(mask)
+    return int(out[0]), [int(x) for x in out[1:]]
+
+
+def hit_dtype(mask):
+    """numpy dtype with the byte layout of HitReg for `mask` (absent fields omitted)."""
+    size, off = layout(mask)
+    names, formats, offsets = [], [], []
+    for name, o, f in zip(("u", "v", "t", "primitive_id", "valid", "px", "py", "pz"), off,
+                          ("<f4", "<f4", "<f4", "<u4", "u1", "<f4", "<f4", "<f4")):
+        if o >= 0:
+            names.append(name)
+            formats.append(f)
+            offsets.append(o)
+    return np.dtype(dict(names=names, formats=formats, offsets=offsets, itemsize=size))
+
+
+def probe():
+    """(number of gfx950 devices, name of the first)."""
+    n = ctypes.c_int(0)
+    buf = ctypes.create_string_buffer(256)
+    lib().prt_mi355x_probe(ctypes.byref(n), buf, 256)
+    return n.value, buf.value.decode()
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if isinstance(a, np.ndarray) else ctypes.c_void_p(a)
+
+
+class Context:
+    """One device context of the C ABI: stream, BVH, staging buffers."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().prt_mi355x_create(device, ctypes.byref(h))
+        if rc:
+            raise PrtError(rc, f"no usable gfx950 device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().prt_mi355x_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc:
+            raise PrtError(rc, lib().prt_mi355x_last_error(self.h).decode())
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        self._check(lib().prt_mi355x_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def set_option(self, name, value):
+        self._check(lib().prt_mi355x_set_option(self.h, OPT[name], int(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        self._check(lib().prt_mi355x_get_option(self.h, OPT[name], ctypes.byref(v)))
+        return v.value
+
+    def stat(self, name):
+        v = ctypes.c_double()
+        self._check(lib().prt_mi355x_get_stat(self.h, STAT[name], ctypes.byref(v)))
+        return v.value
+
+    def set_tris(self, tris):
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self._check(lib().prt_mi355x_set_tris(self.h, _p(tris), len(tris)))
+
+    def set_tris_device(self, ptr, n):
+        self._check(lib().prt_mi355x_set_tris_device(self.h, ctypes.c_void_p(ptr), n))
+
+    def nearest_hits(self, rays, mask=mask_of(ALL_TAGS)):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        size, off = layout(mask)
+        out = np.zeros(len(rays), hit_dtype(mask))
+        offs = np.asarray(off, np.int32)
+        self._check(lib().prt_mi355x_nearest_hits(self.h, _p(rays), len(rays), mask, _p(out), size, _p(offs)))
+        return out
+
+    def nearest_hits_device(self, rays_ptr, n, mask, out_ptr, stream=None):
+        size, off = layout(mask)
+        offs = np.asarray(off, np.int32)
+        self._check(lib().prt_mi355x_nearest_hits_device(self.h, ctypes.c_void_p(rays_ptr), n, mask,
+                                                         ctypes.c_void_p(out_ptr), size, _p(offs),
+                                                         ctypes.c_void_p(stream or 0)))
+
+    def synchronize(self):
+        self._check(lib().prt_mi355x_synchronize(self.h))
+
+    def export(self, which):
+        """Host copy of the device BVH: 'nodes_q', 'nodes_f' or 'tris' (raw bytes)."""
+        w = {"nodes_q": 0, "nodes_f": 1, "tris": 2}[which]
+        n = ctypes.c_uint64()
+        self._check(lib().prt_mi355x_export(self.h, w, None, ctypes.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        if n.value:
+            self._check(lib().prt_mi355x_export(self.h, w, _p(out), ctypes.byref(n)))
+        return out
+
+
+# ------------------------------------------------------------------ reference-shaped API
+
+class Backend:
+    """backend.hpp:12-29: name(), set_tris, is_available, init, shutdown, device_name."""
+
+    def __init__(self, name):
+        self._name = name
+
+    def name(self):
+        return self._name
+
+
+class MI355XBackend(Backend):
+    def __init__(self, device=0):
+        super().__init__("MI355X")
+        self.device = device
+        self.ctx = None
+
+    def is_available(self):
+        try:
+            n, _ = probe()
+        except (OSError, RuntimeError):
+            return False
+        return n > self.device
+
+    def init(self):
+        if self.ctx is None:
+            self.ctx = Context(self.device)
+
+    def shutdown(self):
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
+
+    def device_name(self):
+        self.init()
+        return self.ctx.device_name()
+
+    def set_tris(self, tris):
+        self.init()
+        self.ctx.set_tris(tris)
+
+    def nearest_hits(self, rays, *tags):
+        self.init()
+        return self.ctx.nearest_hits(rays, mask_of(tags if tags else ALL_TAGS))
+
+
+_all = []
+_available = []
+_selected = None
+
+
+def _register(b):
+    """RegisterBackend (backend.hpp:85-95): first available backend becomes selected."""
+    global _selected
+    if b.is_available():
+        if _selected is None:
+            select_backend(b)
+        _available.append(b)
+    _all.append(b)
+
+
+def all_backends():
+    if not _all:
+        _register(MI355XBackend())
+    return list(_all)
+
+
+def available_backends():
+    all_backends()
+    return list(_available)
+
+
+def select_backend(b):
+    """select_backend (src/backend.cpp:50-56): shutdown the old, init the new."""
+    global _selected
+    if _selected is not None:
+        _selected.shutdown()
+    _selected = b
+    b.init()
+
+
+def selected_backend():
+    all_backends()
+    return _selected
+
+
+def nearest_hits(rays, *tags):
+    """Free nearest_hits (nearest_hits_impl.hpp:28-36): dispatch to the selected backend;
+    no tags means all tags (backend.hpp:77-79)."""
+    b = selected_backend()
+    if b is None:
+        raise RuntimeError("Unknown backend")
+    return b.nearest_hits(rays, *tags)

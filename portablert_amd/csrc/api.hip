@@ -1,0 +1,377 @@
+// C ABI of the MI355X nearest-hit backend (declared in include/prt_mi355x.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/prt_mi355x.h"
+#include "common.h"
+#include "internal.h"
+
+struct prt_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string name;
+	std::string err;
+	prt::Bvh8 bvh;
+	prt::BuildScratch bs;
+	prt::BuildParams bp;
+	prt::TraceParams tp;
+	prt::TraceScratch ts;
+	prt::DevBuf rays_stage, out_stage;
+	bool has_scene = false;
+	int time_kernel = 0;
+	int margin_log2 = 10;
+	int64_t alloc_limit = 0;
+	double build_ms = 0.0, trace_ms = 0.0, api_ms = 0.0;
+	prt::TraceStats stats;
+};
+
+namespace {
+
+int fail(prt_ctx *c, int code, const std::string &msg) {
+	if (c) c->err = msg;
+	return code;
+}
+
+int hip_fail(prt_ctx *c, hipError_t e, const char *what) {
+	const int code = (e == hipErrorOutOfMemory) ? PRT_ERR_OOM : PRT_ERR_HIP;
+	return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool is_gfx950(int dev, std::string *name) {
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return false;
+	if (name) *name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+	return true;
+}
+
+// Checks the caller's record layout against HitReg for the mask.
+int check_layout(prt_ctx *c, uint32_t mask, uint32_t stride, const int32_t off[8]) {
+	if (mask > 31u) return fail(c, PRT_ERR_ARG, "field mask has bits above 'valid'");
+	const prt::Layout L = prt::layout_of(mask);
+	if (stride != static_cast<uint32_t>(L.size))
+		return fail(c, PRT_ERR_LAYOUT, "stride " + std::to_string(stride) + " != sizeof(HitReg) " + std::to_string(L.size));
+	if (off)
+		for (int k = 0; k < 8; ++k)
+			if (off[k] != L.off[k])
+				return fail(c, PRT_ERR_LAYOUT, "field offset " + std::to_string(k) + " is " + std::to_string(off[k]) +
+				                                   ", HitReg has " + std::to_string(L.off[k]));
+	return PRT_OK;
+}
+
+void apply_margin(prt_ctx *c) {
+	const float *b = c->bvh.scene_box;
+	float ext = 0.0f;
+	for (int k = 0; k < 3; ++k) ext = std::max(ext, b[3 + k] - b[k]);
+	if (!std::isfinite(ext)) ext = 0.0f;
+	c->tp.margin_rel = std::ldexp(1.0f, -c->margin_log2);
+	// absolute part for hits near t = 0: a fixed fraction of the scene extent
+	c->tp.margin_abs = std::ldexp(ext, -c->margin_log2 - 6);
+}
+
+int do_build(prt_ctx *c, uint64_t n) {
+	bool bad = false;
+	double ms = 0.0;
+	hipError_t e = prt::build_bvh8(c->bs, n, c->bp, c->bvh, c->stream, &bad, &ms);
+	if (e != hipSuccess) {
+		c->has_scene = false;
+		return hip_fail(c, e, "BVH build");
+	}
+	if (bad) {
+		c->has_scene = false;
+		return fail(c, PRT_ERR_INPUT, "set_tris: a triangle vertex is not finite");
+	}
+	c->build_ms = ms;
+	c->has_scene = true;
+	apply_margin(c);
+	return PRT_OK;
+}
+
+int over_limit(prt_ctx *c, uint64_t bytes, const char *what) {
+	if (c->alloc_limit > 0 && bytes > static_cast<uint64_t>(c->alloc_limit))
+		return fail(c, PRT_ERR_OOM, std::string(what) + ": " + std::to_string(bytes) + " bytes over the allocation limit");
+	return PRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prt_mi355x_probe(int *ndev, char *name, size_t cap) {
+	if (!ndev) return PRT_ERR_ARG;
+	*ndev = 0;
+	if (name && cap) name[0] = '\0';
+	int count = 0;
+	if (hipGetDeviceCount(&count) != hipSuccess) {
+		(void)hipGetLastError();
+		return PRT_OK;
+	}
+	std::string first;
+	for (int d = 0; d < count; ++d) {
+		std::string nm;
+		if (is_gfx950(d, &nm)) {
+			if (*ndev == 0) first = nm;
+			++*ndev;
+		}
+	}
+	if (name && cap) std::snprintf(name, cap, "%s", first.c_str());
+	return PRT_OK;
+}
+
+int prt_mi355x_create(int device, prt_ctx **out) {
+	if (!out) return PRT_ERR_ARG;
+	*out = nullptr;
+	int count = 0;
+	if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+		(void)hipGetLastError();
+		return PRT_ERR_NODEV;
+	}
+	std::string nm;
+	if (!is_gfx950(device, &nm)) return PRT_ERR_NODEV;
+	if (hipSetDevice(device) != hipSuccess) return PRT_ERR_HIP;
+	prt_ctx *c = new prt_ctx();
+	c->device = device;
+	c->name = nm;
+	// default priority (no priority streams anywhere in this library)
+	if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+		delete c;
+		return PRT_ERR_HIP;
+	}
+	*out = c;
+	return PRT_OK;
+}
+
+void prt_mi355x_destroy(prt_ctx *c) {
+	if (!c) return;
+	(void)hipSetDevice(c->device);
+	(void)hipStreamSynchronize(c->stream);
+	c->bvh.nodes_q.release();
+	c->bvh.nodes_f.release();
+	c->bvh.tris.release();
+	prt::DevBuf *bufs[] = {&c->bs.tri_in, &c->bs.tbox, &c->bs.keys[0], &c->bs.keys[1], &c->bs.vals[0],
+	                       &c->bs.vals[1], &c->bs.hist, &c->bs.bin, &c->bs.bbox, &c->bs.bin_of,
+	                       &c->bs.tri_dest, &c->bs.misc, &c->ts.counter, &c->ts.spill, &c->rays_stage,
+	                       &c->out_stage};
+	for (auto *b : bufs) b->release();
+	if (c->bs.host_misc) (void)hipHostFree(c->bs.host_misc);
+	(void)hipStreamDestroy(c->stream);
+	delete c;
+}
+
+const char *prt_mi355x_last_error(const prt_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int prt_mi355x_device_name(const prt_ctx *c, char *buf, size_t cap) {
+	if (!c || !buf || !cap) return PRT_ERR_ARG;
+	std::snprintf(buf, cap, "%s", c->name.c_str());
+	return PRT_OK;
+}
+
+int prt_mi355x_set_tris(prt_ctx *c, const float *tris, uint64_t n) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (!tris && n) return fail(c, PRT_ERR_ARG, "set_tris: null triangle pointer");
+	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+	if (int r = over_limit(c, n * 36ull, "set_tris")) return r;
+	hipError_t e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
+	if (e != hipSuccess) return hip_fail(c, e, "set_tris: triangle buffer");
+	if (n) {
+		e = hipMemcpyAsync(c->bs.tri_in.ptr, tris, n * 36ull, hipMemcpyHostToDevice, c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "set_tris: copy");
+	}
+	return do_build(c, n);
+}
+
+int prt_mi355x_set_tris_device(prt_ctx *c, const float *d_tris, uint64_t n) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (!d_tris && n) return fail(c, PRT_ERR_ARG, "set_tris_device: null triangle pointer");
+	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+	if (int r = over_limit(c, n * 36ull, "set_tris")) return r;
+	hipError_t e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
+	if (e != hipSuccess) return hip_fail(c, e, "set_tris: triangle buffer");
+	if (n) {
+		e = hipMemcpyAsync(c->bs.tri_in.ptr, d_tris, n * 36ull, hipMemcpyDeviceToDevice, c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "set_tris: copy");
+	}
+	return do_build(c, n);
+}
+
+static int trace_common(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t mask, void *d_out, hipStream_t st) {
+	float ms = 0.0f;
+	prt::TraceStats stats;
+	hipError_t e = prt::trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st,
+	                               c->time_kernel ? &ms : nullptr, c->tp.count_stats ? &stats : nullptr);
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: traversal");
+	if (c->time_kernel) c->trace_ms = ms;
+	if (c->tp.count_stats) c->stats = stats;
+	return PRT_OK;
+}
+
+int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride,
+                            const int32_t off[8]) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (int r = check_layout(c, mask, stride, off)) return r;
+	if (!c->has_scene) return fail(c, PRT_ERR_STATE, "nearest_hits called before set_tris");
+	if (n == 0) return PRT_OK;
+	if (!rays || !out) return fail(c, PRT_ERR_ARG, "nearest_hits: null pointer");
+	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+	const auto t0 = std::chrono::steady_clock::now();
+	if (int r = over_limit(c, n * (24ull + stride), "nearest_hits")) return r;
+	hipError_t e = c->rays_stage.ensure(n * 24ull);
+	if (e == hipSuccess) e = c->out_stage.ensure(n * static_cast<uint64_t>(stride));
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: staging buffers");
+	e = hipMemcpyAsync(c->rays_stage.ptr, rays, n * 24ull, hipMemcpyHostToDevice, c->stream);
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray copy");
+	if (int r = trace_common(c, c->rays_stage.as<float>(), n, mask, c->out_stage.ptr, c->stream)) return r;
+	e = hipMemcpyAsync(out, c->out_stage.ptr, n * static_cast<uint64_t>(stride), hipMemcpyDeviceToHost, c->stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: result copy");
+	c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	return PRT_OK;
+}
+
+int prt_mi355x_nearest_hits_device(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t mask, void *d_out,
+                                   uint32_t stride, const int32_t off[8], void *stream) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (int r = check_layout(c, mask, stride, off)) return r;
+	if (!c->has_scene) return fail(c, PRT_ERR_STATE, "nearest_hits called before set_tris");
+	if (n == 0) return PRT_OK;
+	if (!d_rays || !d_out) return fail(c, PRT_ERR_ARG, "nearest_hits_device: null pointer");
+	if (reinterpret_cast<uintptr_t>(d_rays) % 4 || reinterpret_cast<uintptr_t>(d_out) % 16)
+		return fail(c, PRT_ERR_ARG, "nearest_hits_device: rays need 4-byte, records 16-byte alignment");
+	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+	return trace_common(c, d_rays, n, mask, d_out, st);
+}
+
+int prt_mi355x_synchronize(prt_ctx *c) {
+	if (!c) return PRT_ERR_ARG;
+	hipError_t e = hipStreamSynchronize(c->stream);
+	return e == hipSuccess ? PRT_OK : hip_fail(c, e, "synchronize");
+}
+
+int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
+	if (!c) return PRT_ERR_ARG;
+	auto bad = [&]() { return fail(c, PRT_ERR_ARG, "option " + std::to_string(opt) + " value " + std::to_string(v)); };
+	switch (opt) {
+	case PRT_OPT_NODE_FORMAT:
+		if (v != 0 && v != 1) return bad();
+		c->tp.node_format = static_cast<int>(v);
+		break;
+	case PRT_OPT_CULL:
+		if (v != 0 && v != 1) return bad();
+		c->tp.cull = static_cast<int>(v);
+		break;
+	case PRT_OPT_LEAF_SIZE:
+		if (v < 1 || v > prt::kMaxLeaf) return bad();
+		c->bp.leaf_size = static_cast<int>(v);
+		break;
+	case PRT_OPT_BLOCK:
+		if (v != 64 && v != 128 && v != 256) return bad();
+		c->tp.block = static_cast<int>(v);
+		break;
+	case PRT_OPT_WAVES_PER_CU:
+		if (v < 1 || v > 64) return bad();
+		c->tp.waves_per_cu = static_cast<int>(v);
+		break;
+	case PRT_OPT_LDS_STACK:
+		if (v != 4 && v != 8 && v != 16) return bad();
+		c->tp.lds_stack = static_cast<int>(v);
+		break;
+	case PRT_OPT_FETCH_MIN:
+		if (v < 1 || v > 64) return bad();
+		c->tp.fetch_min = static_cast<int>(v);
+		break;
+	case PRT_OPT_STEPS:
+		if (v < 1 || v > 4096) return bad();
+		c->tp.steps = static_cast<int>(v);
+		break;
+	case PRT_OPT_COUNT_STATS: c->tp.count_stats = v ? 1 : 0; break;
+	case PRT_OPT_TIME_KERNEL: c->time_kernel = v ? 1 : 0; break;
+	case PRT_OPT_ALLOC_LIMIT:
+		if (v < 0) return bad();
+		c->alloc_limit = v;
+		break;
+	case PRT_OPT_MARGIN_REL_LOG2:
+		if (v < 4 || v > 30) return bad();
+		c->margin_log2 = static_cast<int>(v);
+		apply_margin(c);
+		break;
+	default: return fail(c, PRT_ERR_ARG, "unknown option " + std::to_string(opt));
+	}
+	return PRT_OK;
+}
+
+int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
+	if (!c || !v) return PRT_ERR_ARG;
+	switch (opt) {
+	case PRT_OPT_NODE_FORMAT: *v = c->tp.node_format; break;
+	case PRT_OPT_CULL: *v = c->tp.cull; break;
+	case PRT_OPT_LEAF_SIZE: *v = c->bp.leaf_size; break;
+	case PRT_OPT_BLOCK: *v = c->tp.block; break;
+	case PRT_OPT_WAVES_PER_CU: *v = c->tp.waves_per_cu; break;
+	case PRT_OPT_LDS_STACK: *v = c->tp.lds_stack; break;
+	case PRT_OPT_FETCH_MIN: *v = c->tp.fetch_min; break;
+	case PRT_OPT_STEPS: *v = c->tp.steps; break;
+	case PRT_OPT_COUNT_STATS: *v = c->tp.count_stats; break;
+	case PRT_OPT_TIME_KERNEL: *v = c->time_kernel; break;
+	case PRT_OPT_ALLOC_LIMIT: *v = c->alloc_limit; break;
+	case PRT_OPT_MARGIN_REL_LOG2: *v = c->margin_log2; break;
+	default: return PRT_ERR_ARG;
+	}
+	return PRT_OK;
+}
+
+int prt_mi355x_get_stat(const prt_ctx *c, int stat, double *v) {
+	if (!c || !v) return PRT_ERR_ARG;
+	switch (stat) {
+	case PRT_STAT_BUILD_MS: *v = c->build_ms; break;
+	case PRT_STAT_TRACE_MS: *v = c->trace_ms; break;
+	case PRT_STAT_NODES: *v = c->bvh.n_nodes; break;
+	case PRT_STAT_DEPTH: *v = c->bvh.depth; break;
+	case PRT_STAT_NODE_VISITS: *v = static_cast<double>(c->stats.node_visits); break;
+	case PRT_STAT_TRI_TESTS: *v = static_cast<double>(c->stats.tri_tests); break;
+	case PRT_STAT_LEAF_VISITS: *v = static_cast<double>(c->stats.leaf_visits); break;
+	case PRT_STAT_NODE_BYTES: *v = c->tp.node_format == 0 ? sizeof(prt::NodeF) : sizeof(prt::NodeQ); break;
+	case PRT_STAT_API_MS: *v = c->api_ms; break;
+	case PRT_STAT_N_TRIS: *v = static_cast<double>(c->bvh.n_tris); break;
+	default: return PRT_ERR_ARG;
+	}
+	return PRT_OK;
+}
+
+int prt_mi355x_layout(uint32_t mask, int32_t out[9]) {
+	if (!out || mask > 31u) return PRT_ERR_ARG;
+	const prt::Layout L = prt::layout_of(mask);
+	out[0] = L.size;
+	for (int k = 0; k < 8; ++k) out[1 + k] = L.off[k];
+	return PRT_OK;
+}
+
+int prt_mi355x_export(prt_ctx *c, int which, void *out, uint64_t *bytes) {
+	if (!c || !bytes) return PRT_ERR_ARG;
+	const prt::DevBuf *b = nullptr;
+	uint64_t sz = 0;
+	switch (which) {
+	case 0: b = &c->bvh.nodes_q; sz = sizeof(prt::NodeQ) * uint64_t(c->bvh.n_nodes); break;
+	case 1: b = &c->bvh.nodes_f; sz = sizeof(prt::NodeF) * uint64_t(c->bvh.n_nodes); break;
+	case 2: b = &c->bvh.tris; sz = sizeof(prt::TriRec) * c->bvh.n_tris; break;
+	default: return fail(c, PRT_ERR_ARG, "export: unknown buffer");
+	}
+	if (!c->has_scene) sz = 0;
+	*bytes = sz;
+	if (!out || sz == 0) return PRT_OK;
+	hipError_t e = hipMemcpyAsync(out, b->ptr, sz, hipMemcpyDeviceToHost, c->stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+	return e == hipSuccess ? PRT_OK : hip_fail(c, e, "export");
+}
+
+}  // extern "C"

@@ -1,0 +1,644 @@
+// GPU BVH8 build for set_tris (replaces the reference's host binned-SAH BVH2,
+// include/portableRT/bvh.hpp:58-193, whose result does not matter for the nearest hit except
+// for tie order).
+//
+// Pipeline, all on one stream:
+//   1. k_tri_prep     triangle boxes, centroid bounds, finiteness check
+//   2. k_morton       30-bit Morton code of the box centre (Karras 2012)
+//   3. radix sort     stable LSD, 8-bit digits, hand-written (hist / scan / scatter)
+//   4. k_karras       binary radix tree over the sorted codes (Karras 2012), index tiebreak
+//   5. k_bottomup     boxes and triangle counts leaf to root, second arrival continues
+//   6. k_collapse     per BVH8 level: open the child of largest surface area until 8 children,
+//                     subtrees of <= leaf_size triangles become leaves; writes fp32 and
+//                     8-bit quantized nodes (bounds rounded outward and verified in fp32)
+//   7. k_emit_tris    leaf triangle records in the order the nodes address them
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "common.h"
+#include "internal.h"
+
+namespace prt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 16;                       // keys per thread per radix tile
+constexpr int kSortTile = kBlock * kSortItems;       // 4096 keys per tile
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// misc buffer layout (uint32 words)
+enum : int {
+	M_CB = 0,          // 6 words: ordered centroid min xyz, max xyz
+	M_ERR = 6,         // non-finite input flag
+	M_NODES = 7,       // BVH8 node allocation counter
+	M_TRIS = 8,        // leaf triangle slot counter
+	M_LVL_BEGIN = 9,   // current level [begin, end)
+	M_LVL_END = 10,
+	M_DEPTH = 11,
+	M_SB = 12,         // 6 words: ordered scene box min xyz, max xyz
+	M_WORDS = 24
+};
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+	const uint32_t u = __float_as_uint(f);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) {
+	return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_init_misc(uint32_t *misc) {
+	const int i = threadIdx.x;
+	if (i < M_WORDS) {
+		uint32_t v = 0;
+		if (i < 3 || (i >= M_SB && i < M_SB + 3)) v = f2o(__builtin_inff());
+		else if (i < 6 || (i >= M_SB + 3 && i < M_SB + 6)) v = f2o(-__builtin_inff());
+		misc[i] = v;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ tris, uint32_t n,
+                                                     float4 *__restrict__ tbox, uint32_t *misc) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	float c[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+	float cmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	float smn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+	float smx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	if (i < n) {
+		float v[9];
+#pragma unroll
+		for (int k = 0; k < 9; ++k) v[k] = tris[9ull * i + k];
+		bool finite = true;
+#pragma unroll
+		for (int k = 0; k < 9; ++k) finite &= __builtin_isfinite(v[k]);
+		if (!finite) atomicOr(misc + M_ERR, 1u);
+		float lo[3], hi[3];
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			lo[k] = fminf(v[k], fminf(v[3 + k], v[6 + k]));
+			hi[k] = fmaxf(v[k], fmaxf(v[3 + k], v[6 + k]));
+			c[k] = cmax[k] = 0.5f * lo[k] + 0.5f * hi[k];
+			smn[k] = lo[k];
+			smx[k] = hi[k];
+		}
+		tbox[2ull * i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+		tbox[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+	}
+	// wave reduction, then one atomic per wave and axis
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		for (int off = 32; off > 0; off >>= 1) {
+			c[k] = fminf(c[k], __shfl_xor(c[k], off));
+			cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off));
+			smn[k] = fminf(smn[k], __shfl_xor(smn[k], off));
+			smx[k] = fmaxf(smx[k], __shfl_xor(smx[k], off));
+		}
+	}
+	if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			if (c[k] <= cmax[k]) {
+				atomicMin(misc + M_CB + k, f2o(c[k]));
+				atomicMax(misc + M_CB + 3 + k, f2o(cmax[k]));
+				atomicMin(misc + M_SB + k, f2o(smn[k]));
+				atomicMax(misc + M_SB + 3 + k, f2o(smx[k]));
+			}
+		}
+	}
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+	x &= 0x3FFu;
+	x = (x | (x << 16)) & 0x030000FFu;
+	x = (x | (x << 8)) & 0x0300F00Fu;
+	x = (x | (x << 4)) & 0x030C30C3u;
+	x = (x | (x << 2)) & 0x09249249u;
+	return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float4 *__restrict__ tbox, uint32_t n,
+                                                   const uint32_t *misc, uint64_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ vals) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	const float4 lo = tbox[2ull * i], hi = tbox[2ull * i + 1];
+	const float c[3] = {0.5f * lo.x + 0.5f * hi.x, 0.5f * lo.y + 0.5f * hi.y, 0.5f * lo.z + 0.5f * hi.z};
+	uint32_t q[3];
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		const float mn = o2f(misc[M_CB + k]), mx = o2f(misc[M_CB + 3 + k]);
+		const float ext = mx - mn;
+		float x = ext > 0.0f ? (c[k] - mn) / ext : 0.0f;
+		x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+		q[k] = static_cast<uint32_t>(x);
+	}
+	keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+	vals[i] = i;
+}
+
+// ------------------------------------------------------------------------- radix sort
+
+__global__ __launch_bounds__(kBlock) void k_hist(const uint64_t *__restrict__ keys, uint32_t n, int shift,
+                                                 uint32_t *__restrict__ hist, uint32_t tiles) {
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+	for (int r = 0; r < kSortItems; ++r) {
+		const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
+		if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+	}
+	__syncthreads();
+	hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of hist (digit-major: entry d * tiles + t) in place, one workgroup.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t *hist, uint32_t total) {
+	__shared__ uint32_t part[1024];
+	const uint32_t per = (total + 1023) / 1024;
+	const uint32_t b = threadIdx.x * per;
+	const uint32_t e = min(total, b + per);
+	uint32_t s = 0;
+	for (uint32_t i = b; i < e; ++i) s += hist[i];
+	part[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		const uint32_t v = threadIdx.x >= static_cast<uint32_t>(off) ? part[threadIdx.x - off] : 0u;
+		__syncthreads();
+		part[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint32_t run = part[threadIdx.x] - s;
+	for (uint32_t i = b; i < e; ++i) {
+		const uint32_t x = hist[i];
+		hist[i] = run;
+		run += x;
+	}
+}
+
+// Stable scatter: keys of tile t are ranked in index order (round-major, then wave, then
+// lane); equal digits are matched within a wave with eight ballots.
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                    uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                    uint32_t n, int shift, const uint32_t *__restrict__ hist,
+                                                    uint32_t tiles) {
+	__shared__ uint32_t offs[256];
+	__shared__ uint32_t wcnt[kBlock / 64][256];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	offs[threadIdx.x] = hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x];
+	const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+		for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
+		__syncthreads();
+		const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
+		const bool act = i < n;
+		uint64_t k = 0;
+		uint32_t v = 0, digit = 0;
+		if (act) {
+			k = kin[i];
+			v = vin[i];
+			digit = static_cast<uint32_t>(k >> shift) & 0xFFu;
+		}
+		uint64_t peers = __ballot(act);
+#pragma unroll
+		for (int b = 0; b < 8; ++b) {
+			const bool bit = (digit >> b) & 1u;
+			const uint64_t m = __ballot(bit);
+			peers &= bit ? m : ~m;
+		}
+		const uint32_t rank = __popcll(peers & lt);
+		if (act && rank == 0) wcnt[wave][digit] = __popcll(peers);
+		__syncthreads();
+		uint32_t pos = 0;
+		if (act) {
+			pos = offs[digit] + rank;
+			for (int w = 0; w < wave; ++w) pos += wcnt[w][digit];
+		}
+		__syncthreads();
+		uint32_t add = 0;
+#pragma unroll
+		for (int w = 0; w < kBlock / 64; ++w) add += wcnt[w][threadIdx.x];
+		offs[threadIdx.x] += add;
+		if (act) {
+			kout[pos] = k;
+			vout[pos] = v;
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------------------ Karras 2012
+
+struct BinTree {
+	uint32_t *left, *right, *parent, *first, *count, *flag;
+	float4 *box;  // 2 per node: lo, hi
+	uint32_t n;   // leaves; internal nodes 0..n-2, leaf j is node (n-1)+j
+};
+
+__device__ __forceinline__ int delta(const uint64_t *keys, uint32_t n, int i, int j) {
+	if (j < 0 || j >= static_cast<int>(n)) return -1;
+	const uint64_t a = keys[i], b = keys[j];
+	if (a != b) return __clzll(a ^ b);
+	return 64 + __clz(static_cast<uint32_t>(i) ^ static_cast<uint32_t>(j));
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint64_t *__restrict__ keys, BinTree t) {
+	const int n = static_cast<int>(t.n);
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n - 1) return;
+	const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+	const int dmin = delta(keys, n, i, i - d);
+	int lmax = 2;
+	while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+	int l = 0;
+	for (int s = lmax >> 1; s >= 1; s >>= 1)
+		if (delta(keys, n, i, i + (l + s) * d) > dmin) l += s;
+	const int j = i + l * d;
+	const int dnode = delta(keys, n, i, j);
+	int s = 0, div = 2, step;
+	do {
+		step = (l + div - 1) / div;
+		if (delta(keys, n, i, i + (s + step) * d) > dnode) s += step;
+		div <<= 1;
+	} while (step > 1);
+	const int gamma = i + s * d + min(d, 0);
+	const uint32_t lc = (min(i, j) == gamma) ? static_cast<uint32_t>(n - 1 + gamma) : static_cast<uint32_t>(gamma);
+	const uint32_t rc = (max(i, j) == gamma + 1) ? static_cast<uint32_t>(n - 1 + gamma + 1) : static_cast<uint32_t>(gamma + 1);
+	t.left[i] = lc;
+	t.right[i] = rc;
+	t.parent[lc] = static_cast<uint32_t>(i);
+	t.parent[rc] = static_cast<uint32_t>(i);
+	t.first[i] = static_cast<uint32_t>(min(i, j));
+	t.flag[i] = 0;
+	if (i == 0) t.parent[0] = kNone;
+}
+
+__device__ __forceinline__ float ld_coherent(const float *p) {
+	return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED,
+	                                         __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ tbox,
+                                                     const uint32_t *__restrict__ sorted, BinTree t) {
+	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	if (j >= t.n) return;
+	uint32_t node = t.n - 1 + j;
+	const uint32_t tri = sorted[j];
+	t.box[2ull * node] = tbox[2ull * tri];
+	t.box[2ull * node + 1] = tbox[2ull * tri + 1];
+	t.count[node] = 1;
+	t.first[node] = j;
+	__threadfence();
+	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
+	while (p != kNone) {
+		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
+		__threadfence();
+		const uint32_t a = t.left[p], b = t.right[p];
+		const float *ba = reinterpret_cast<const float *>(t.box + 2ull * a);
+		const float *bb = reinterpret_cast<const float *>(t.box + 2ull * b);
+		float4 lo, hi;
+		lo.x = fminf(ld_coherent(ba + 0), ld_coherent(bb + 0));
+		lo.y = fminf(ld_coherent(ba + 1), ld_coherent(bb + 1));
+		lo.z = fminf(ld_coherent(ba + 2), ld_coherent(bb + 2));
+		lo.w = 0.0f;
+		hi.x = fmaxf(ld_coherent(ba + 4), ld_coherent(bb + 4));
+		hi.y = fmaxf(ld_coherent(ba + 5), ld_coherent(bb + 5));
+		hi.z = fmaxf(ld_coherent(ba + 6), ld_coherent(bb + 6));
+		hi.w = 0.0f;
+		t.box[2ull * p] = lo;
+		t.box[2ull * p + 1] = hi;
+		t.count[p] = ld_coherent(t.count + a) + ld_coherent(t.count + b);
+		__threadfence();
+		p = t.parent[p];
+	}
+}
+
+// ------------------------------------------------------------------------- BVH8 collapse
+
+__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
+	const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+	return dx * dy + dx * dz + dy * dz;
+}
+
+// Quantizes [clo, chi] against a grid at p with step s = 2^e; both bounds round outward,
+// checked with the same fmaf the traversal evaluates.
+__device__ __forceinline__ void quantize(float clo, float chi, float p, float s, uint8_t &qlo, uint8_t &qhi) {
+	int lo = static_cast<int>(floorf((clo - p) / s));
+	lo = max(0, min(255, lo));
+	while (lo > 0 && fmaf(static_cast<float>(lo), s, p) > clo) --lo;
+	int hi = static_cast<int>(ceilf((chi - p) / s));
+	hi = max(0, min(255, hi));
+	while (hi < 255 && fmaf(static_cast<float>(hi), s, p) < chi) ++hi;
+	qlo = static_cast<uint8_t>(lo);
+	qhi = static_cast<uint8_t>(hi);
+}
+
+// Smallest e in [-126, 127] with p + 255 * 2^e >= top in fp32.
+__device__ __forceinline__ int grid_exp(float p, float top) {
+	const float ext = top - p;
+	int e = -126;
+	if (ext > 0.0f) {
+		int ex;
+		(void)frexpf(ext / 255.0f, &ex);  // ext/255 in [2^(ex-1), 2^ex)
+		e = max(-126, min(127, ex - 1));
+	}
+	while (e < 127 && fmaf(255.0f, exp2i(e), p) < top) ++e;
+	return e;
+}
+
+struct CollapseArgs {
+	BinTree t;
+	uint32_t *misc;
+	uint32_t *bin_of;
+	uint32_t *tri_dest;
+	NodeQ *nq;
+	NodeF *nf;
+	int leaf_size;
+};
+
+__global__ __launch_bounds__(kBlock) void k_level_advance(uint32_t *misc) {
+	if (threadIdx.x == 0) {
+		const uint32_t end = misc[M_LVL_END];
+		const uint32_t nodes = misc[M_NODES];
+		misc[M_LVL_BEGIN] = end;
+		misc[M_LVL_END] = nodes;
+		if (nodes > end) misc[M_DEPTH] += 1;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
+	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
+	const BinTree &t = a.t;
+	const uint32_t n_int = t.n - 1;  // binary internal nodes are ids < n_int
+	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
+		const uint32_t b = a.bin_of[i];
+		uint32_t el[8];
+		float ar[8];
+		int ne;
+		if (b < n_int) {
+			el[0] = t.left[b];
+			el[1] = t.right[b];
+			ne = 2;
+		} else {
+			el[0] = b;
+			ne = 1;
+		}
+#pragma unroll
+		for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
+		for (int k = 0; k < ne; ++k)
+			if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
+		while (ne < 8) {
+			int best = -1;
+			float ba = -1.0f;
+#pragma unroll
+			for (int k = 0; k < 8; ++k)
+				if (k < ne && ar[k] > ba) {
+					ba = ar[k];
+					best = k;
+				}
+			if (best < 0) break;
+			const uint32_t x = el[best];
+			const uint32_t l = t.left[x], r = t.right[x];
+			el[best] = l;
+			ar[best] = (l < n_int) ? half_area(t.box[2ull * l], t.box[2ull * l + 1]) : -1.0f;
+			el[ne] = r;
+			ar[ne] = (r < n_int) ? half_area(t.box[2ull * r], t.box[2ull * r + 1]) : -1.0f;
+			++ne;
+		}
+		// classify: subtrees of at most leaf_size triangles become leaves
+		uint32_t cnt[8];
+		int n_inner = 0;
+		uint32_t n_leaf_tris = 0;
+		for (int k = 0; k < ne; ++k) {
+			cnt[k] = t.count[el[k]];
+			if (cnt[k] > static_cast<uint32_t>(a.leaf_size)) ++n_inner;
+			else n_leaf_tris += cnt[k];
+		}
+		const uint32_t child_base = n_inner ? atomicAdd(a.misc + M_NODES, static_cast<uint32_t>(n_inner)) : 0u;
+		const uint32_t tri_base = n_leaf_tris ? atomicAdd(a.misc + M_TRIS, n_leaf_tris) : 0u;
+		// node box = union of children
+		float4 nlo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+		float4 nhi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
+		for (int k = 0; k < ne; ++k) {
+			const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
+			nlo.x = fminf(nlo.x, lo.x);
+			nlo.y = fminf(nlo.y, lo.y);
+			nlo.z = fminf(nlo.z, lo.z);
+			nhi.x = fmaxf(nhi.x, hi.x);
+			nhi.y = fmaxf(nhi.y, hi.y);
+			nhi.z = fmaxf(nhi.z, hi.z);
+		}
+		NodeQ q;
+		NodeF f;
+		const float np[3] = {nlo.x, nlo.y, nlo.z}, nt[3] = {nhi.x, nhi.y, nhi.z};
+		float sc[3];
+#pragma unroll
+		for (int ax = 0; ax < 3; ++ax) {
+			const int e = grid_exp(np[ax], nt[ax]);
+			q.p[ax] = np[ax];
+			q.e[ax] = static_cast<int8_t>(e);
+			sc[ax] = exp2i(e);
+		}
+		q.child_base = f.child_base = child_base;
+		q.tri_base = f.tri_base = tri_base;
+		uint8_t imask = 0;
+		uint32_t inner = 0, toff = 0;
+#pragma unroll
+		for (int k = 0; k < 8; ++k) {
+			uint8_t meta = 0;
+			float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+			float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+			if (k < ne) {
+				const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
+				lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
+				hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
+				if (cnt[k] > static_cast<uint32_t>(a.leaf_size)) {
+					meta = static_cast<uint8_t>(kMetaInternal | inner);
+					imask |= static_cast<uint8_t>(1u << k);
+					a.bin_of[child_base + inner] = el[k];
+					++inner;
+				} else {
+					meta = static_cast<uint8_t>((cnt[k] << 5) | toff);
+					const uint32_t first = t.first[el[k]];
+					for (uint32_t m = 0; m < cnt[k]; ++m) a.tri_dest[first + m] = tri_base + toff + m;
+					toff += cnt[k];
+				}
+			}
+			q.meta[k] = f.meta[k] = meta;
+#pragma unroll
+			for (int ax = 0; ax < 3; ++ax) {
+				f.lo[ax][k] = lo3[ax];
+				f.hi[ax][k] = hi3[ax];
+				if (k < ne) {
+					quantize(lo3[ax], hi3[ax], np[ax], sc[ax], q.qlo[ax][k], q.qhi[ax][k]);
+				} else {
+					q.qlo[ax][k] = 255;
+					q.qhi[ax][k] = 0;
+				}
+			}
+		}
+		q.imask = imask;
+		a.nq[i] = q;
+		a.nf[i] = f;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_emit_tris(const float *__restrict__ tris, const uint32_t *__restrict__ sorted,
+                                                      const uint32_t *__restrict__ dest, uint32_t n,
+                                                      TriRec *__restrict__ out) {
+	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	if (j >= n) return;
+	const uint32_t src = sorted[j];
+	TriRec r;
+#pragma unroll
+	for (int k = 0; k < 9; ++k) r.v[k] = tris[9ull * src + k];
+	r.id = src;
+	r.pad[0] = r.pad[1] = 0;
+	out[dest[j]] = r;
+}
+
+__global__ void k_root(uint32_t *misc, uint32_t *bin_of, uint32_t root) {
+	if (threadIdx.x == 0) {
+		bin_of[0] = root;
+		misc[M_NODES] = 1;
+		misc[M_LVL_BEGIN] = 0;
+		misc[M_LVL_END] = 0;
+		misc[M_DEPTH] = 0;
+	}
+}
+
+inline uint32_t blocks_for(uint64_t n, int per = kBlock) { return static_cast<uint32_t>((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 &out, hipStream_t st,
+                      bool *bad_input, double *build_ms) {
+	auto h2f = [](uint32_t u) {
+		union {
+			uint32_t u;
+			float f;
+		} c;
+		c.u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+		return c.f;
+	};
+#define PRT_TRY(x)                                                                                 \
+	do {                                                                                           \
+		hipError_t e_ = (x);                                                                       \
+		if (e_ != hipSuccess) return e_;                                                           \
+	} while (0)
+	*bad_input = false;
+	out.n_tris = n64;
+	out.n_nodes = 0;
+	out.depth = 0;
+	out.leaf_size = static_cast<uint32_t>(p.leaf_size);
+	if (n64 == 0) return hipSuccess;
+	if (n64 > static_cast<uint64_t>(kTriMask)) return hipErrorInvalidValue;
+	const uint32_t n = static_cast<uint32_t>(n64);
+	const uint32_t nb = 2 * n - 1;
+
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	if (build_ms) {
+		PRT_TRY(hipEventCreate(&ev0));
+		PRT_TRY(hipEventCreate(&ev1));
+		PRT_TRY(hipEventRecord(ev0, st));
+	}
+	PRT_TRY(s.tbox.ensure(sizeof(float4) * 2 * n));
+	for (int k = 0; k < 2; ++k) {
+		PRT_TRY(s.keys[k].ensure(sizeof(uint64_t) * n));
+		PRT_TRY(s.vals[k].ensure(sizeof(uint32_t) * n));
+	}
+	const uint32_t tiles = blocks_for(n, kSortTile);
+	PRT_TRY(s.hist.ensure(sizeof(uint32_t) * 256 * tiles));
+	PRT_TRY(s.bin.ensure(sizeof(uint32_t) * (6ull * nb + 8)));
+	PRT_TRY(s.bbox.ensure(sizeof(float4) * 2ull * nb));
+	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * n));
+	PRT_TRY(s.tri_dest.ensure(sizeof(uint32_t) * n));
+	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * M_WORDS));
+	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * M_WORDS));
+	PRT_TRY(out.nodes_q.ensure(sizeof(NodeQ) * n));
+	PRT_TRY(out.nodes_f.ensure(sizeof(NodeF) * n));
+	PRT_TRY(out.tris.ensure(sizeof(TriRec) * n));
+
+	uint32_t *misc = s.misc.as<uint32_t>();
+	const float *tri_in = s.tri_in.as<float>();
+	hipLaunchKernelGGL(k_init_misc, dim3(1), dim3(64), 0, st, misc);
+	hipLaunchKernelGGL(k_tri_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, n, s.tbox.as<float4>(), misc);
+	hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(), n, misc,
+	                   s.keys[0].as<uint64_t>(), s.vals[0].as<uint32_t>());
+	int cur = 0;
+	for (int shift = 0; shift < 30; shift += 8) {
+		hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), n, shift,
+		                   s.hist.as<uint32_t>(), tiles);
+		hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.hist.as<uint32_t>(), 256 * tiles);
+		hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(),
+		                   s.vals[cur].as<uint32_t>(), s.keys[cur ^ 1].as<uint64_t>(), s.vals[cur ^ 1].as<uint32_t>(),
+		                   n, shift, s.hist.as<uint32_t>(), tiles);
+		cur ^= 1;
+	}
+	BinTree t;
+	uint32_t *bw = s.bin.as<uint32_t>();
+	t.left = bw;
+	t.right = bw + nb;
+	t.parent = bw + 2ull * nb;
+	t.first = bw + 3ull * nb;
+	t.count = bw + 4ull * nb;
+	t.flag = bw + 5ull * nb;
+	t.box = s.bbox.as<float4>();
+	t.n = n;
+	if (n > 1)
+		hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), t);
+	hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
+	                   s.vals[cur].as<uint32_t>(), t);
+	// BVH8 levels; the root is binary node 0 (or leaf 0 when n == 1)
+	hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, misc, s.bin_of.as<uint32_t>(), 0u);
+	CollapseArgs ca;
+	ca.t = t;
+	ca.misc = misc;
+	ca.bin_of = s.bin_of.as<uint32_t>();
+	ca.tri_dest = s.tri_dest.as<uint32_t>();
+	ca.nq = out.nodes_q.as<NodeQ>();
+	ca.nf = out.nodes_f.as<NodeF>();
+	ca.leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size));
+	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
+	for (int level = 0; level < 128; ++level) {
+		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
+		const uint32_t grid = std::min<uint32_t>(blocks_for(n), 2048);
+		hipLaunchKernelGGL(k_collapse, dim3(grid), dim3(kBlock), 0, st, ca);
+		if ((level & 3) == 3) {
+			PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * M_WORDS, hipMemcpyDeviceToHost, st));
+			PRT_TRY(hipStreamSynchronize(st));
+			// done when the level just processed created no new node
+			if (hm[M_NODES] == hm[M_LVL_END]) break;
+		}
+	}
+	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(),
+	                   s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>());
+	PRT_TRY(hipGetLastError());
+	PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * M_WORDS, hipMemcpyDeviceToHost, st));
+	if (build_ms) PRT_TRY(hipEventRecord(ev1, st));
+	PRT_TRY(hipStreamSynchronize(st));
+	if (build_ms) {
+		float ms = 0.0f;
+		PRT_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+		*build_ms = ms;
+		(void)hipEventDestroy(ev0);
+		(void)hipEventDestroy(ev1);
+	}
+	*bad_input = hm[M_ERR] != 0;
+	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SB + k]);
+	out.n_nodes = hm[M_NODES];
+	out.depth = hm[M_DEPTH];
+	if (hm[M_TRIS] != n || hm[M_NODES] != hm[M_LVL_END]) return hipErrorUnknown;  // incomplete build
+	return hipSuccess;
+#undef PRT_TRY
+}
+
+}  // namespace prt

@@ -1,0 +1,235 @@
+// Shared device-side definitions of the MI355X nearest-hit backend: BVH8 node formats, the
+// leaf triangle record, the exact reference predicates and the hit-record writer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace prt {
+
+// ----------------------------------------------------------------------------- formats
+
+// Output field bits (filter tags of the reference, hitreg.hpp:48-54).
+enum : uint32_t { F_UV = 1, F_T = 2, F_ID = 4, F_P = 8, F_VALID = 16 };
+
+// Leaf triangle record, 48 bytes: the three vertices as given to set_tris and the
+// primitive id. The exact leaf test reads only this record.
+struct alignas(16) TriRec {
+	float v[9];
+	uint32_t id;
+	uint32_t pad[2];
+};
+static_assert(sizeof(TriRec) == 48, "TriRec");
+
+// meta byte of a BVH8 child slot: 0 empty; internal child: 0xE0 | offset from child_base;
+// leaf child: (count << 5) | offset from tri_base, count in 1..kMaxLeaf.
+constexpr int kMaxLeaf = 4;
+constexpr uint8_t kMetaInternal = 0xE0;
+__host__ __device__ inline bool meta_internal(uint32_t m) { return (m & 0xE0u) == 0xE0u; }
+__host__ __device__ inline uint32_t meta_count(uint32_t m) { return m >> 5; }
+__host__ __device__ inline uint32_t meta_offset(uint32_t m) { return m & 0x1Fu; }
+
+// 8-bit quantized BVH8 node, 80 bytes. Child k's box on axis a is
+// [p_a + qlo_a[k] * 2^e_a, p_a + qhi_a[k] * 2^e_a], evaluated in fp32 as fmaf(q, 2^e, p);
+// the builder verifies each bound rounds outward from the exact child box.
+struct alignas(16) NodeQ {
+	float p[3];
+	int8_t e[3];
+	uint8_t imask;
+	uint32_t child_base;
+	uint32_t tri_base;
+	uint8_t meta[8];
+	uint8_t qlo[3][8];
+	uint8_t qhi[3][8];
+};
+static_assert(sizeof(NodeQ) == 80, "NodeQ");
+
+// fp32 BVH8 node with the same topology, 208 bytes.
+struct alignas(16) NodeF {
+	uint32_t child_base;
+	uint32_t tri_base;
+	uint8_t meta[8];
+	float lo[3][8];
+	float hi[3][8];
+};
+static_assert(sizeof(NodeF) == 208, "NodeF");
+
+// Stack / work entry: bit 31 set = leaf (bits 29..30 = count-1, bits 0..28 = first
+// triangle record); clear = internal node index.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kTriMask = 0x1FFFFFFFu;
+__host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count) {
+	return kLeafBit | ((count - 1u) << 29) | first;
+}
+
+// Scale 2^e as a normal float; e is kept in [-126, 127] by the builder.
+__host__ __device__ inline float exp2i(int e) {
+	union {
+		uint32_t u;
+		float f;
+	} c;
+	c.u = static_cast<uint32_t>(e + 127) << 23;
+	return c.f;
+}
+
+// ----------------------------------------------------------------- reference predicates
+
+// std::min / std::max semantics (the first argument wins when the comparison is false).
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+// Slab test of the reference (bvh.hpp:195-222) on one box, with the reciprocal passed in
+// (1.0f / dir is the same IEEE value in every call). NaN behaviour is reproduced exactly.
+__device__ __forceinline__ bool ref_slab(const float o[3], const float inv[3], const float lo[3],
+                                         const float hi[3]) {
+#pragma clang fp contract(off)
+	const float t1 = (lo[0] - o[0]) * inv[0];
+	const float t2 = (hi[0] - o[0]) * inv[0];
+	const float t3 = (lo[1] - o[1]) * inv[1];
+	const float t4 = (hi[1] - o[1]) * inv[1];
+	const float t5 = (lo[2] - o[2]) * inv[2];
+	const float t6 = (hi[2] - o[2]) * inv[2];
+	const float tmin = smax(smax(smin(t1, t2), smin(t3, t4)), smin(t5, t6));
+	const float tmax = smin(smin(smax(t1, t2), smax(t3, t4)), smax(t5, t6));
+	return !(tmax < 0.0f) && !(tmin > tmax);
+}
+
+// Moller-Trumbore of the reference (core.hpp:27-65), every operation rounded on its own:
+// rejects det == 0 exactly, inclusive u/v bounds, one IEEE division, t without a range check.
+__device__ __forceinline__ bool ref_mt(const float v[9], const float o[3], const float d[3], float &t,
+                                       float &u, float &vv) {
+#pragma clang fp contract(off)
+	const float e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
+	const float e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
+	const float px = d[1] * e2z - d[2] * e2y;
+	const float py = d[2] * e2x - d[0] * e2z;
+	const float pz = d[0] * e2y - d[1] * e2x;
+	const float det = e1x * px + e1y * py + e1z * pz;
+	const float inv = 1.0f / det;
+	const float tx = o[0] - v[0], ty = o[1] - v[1], tz = o[2] - v[2];
+	u = (tx * px + ty * py + tz * pz) * inv;
+	const float qx = ty * e1z - tz * e1y;
+	const float qy = tz * e1x - tx * e1z;
+	const float qz = tx * e1y - ty * e1x;
+	vv = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
+	t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+	// the reference returns early on each failed test; evaluating all terms and testing the
+	// same conditions afterwards gives the same accept/reject and the same t, u, v
+	return !(det == 0.0f) && !(u < 0.0f || u > 1.0f) && !(vv < 0.0f || u + vv > 1.0f);
+}
+
+// Exact candidate test of one triangle: reference slab on its exact box, then the reference
+// leaf test (bvh.hpp:238-246).
+__device__ __forceinline__ bool exact_candidate(const float v[9], const float o[3], const float d[3],
+                                                const float inv[3], float &t, float &u, float &vv) {
+	float lo[3], hi[3];
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		lo[k] = smin(v[k], smin(v[3 + k], v[6 + k]));
+		hi[k] = smax(v[k], smax(v[3 + k], v[6 + k]));
+	}
+	const bool box = ref_slab(o, inv, lo, hi);
+	const bool tri = ref_mt(v, o, d, t, u, vv);
+	return box && tri;
+}
+
+// Keep the better of (t, id): smaller t, equal finite t resolved to the lower id. NaN and +inf
+// never win (the reference's strict t < t_near with t_near starting at +inf).
+__device__ __forceinline__ bool better(float t, uint32_t id, float tb, uint32_t idb) {
+	return (t < tb) || (t == tb && t < __builtin_inff() && id < idb);
+}
+
+// ---------------------------------------------------------------------- hit record writer
+
+// Byte layout of HitReg<Tags...> for a field mask (hitreg.hpp:29-44): members u v t id valid
+// px py pz in that order; absent members are 1-byte empties, present floats/uint32 align to 4,
+// valid is one byte. off[] = u v t id valid px py pz (-1 absent).
+struct Layout {
+	int32_t size;
+	int32_t off[8];
+};
+
+__host__ __device__ constexpr Layout layout_of(uint32_t mask) {
+	Layout L{};
+	int pos = 0, align = 1;
+	auto place = [&](bool present, int sz, int al) {
+		if (!present) {  // Empty member: one byte, alignment 1
+			pos += 1;
+			return -1;
+		}
+		pos = (pos + al - 1) / al * al;
+		const int o = pos;
+		pos += sz;
+		if (al > align) align = al;
+		return o;
+	};
+	const bool uv = mask & F_UV, t = mask & F_T, id = mask & F_ID, p = mask & F_P, va = mask & F_VALID;
+	L.off[0] = place(uv, 4, 4);
+	L.off[1] = place(uv, 4, 4);
+	L.off[2] = place(t, 4, 4);
+	L.off[3] = place(id, 4, 4);
+	L.off[4] = place(va, 1, 1);
+	L.off[5] = place(p, 4, 4);
+	L.off[6] = place(p, 4, 4);
+	L.off[7] = place(p, 4, 4);
+	L.size = (pos + align - 1) / align * align;
+	return L;
+}
+
+struct Hit {
+	float t, u, v;
+	uint32_t id;
+};
+
+// Writes one full record (padding bytes zero) with dword stores. `Mask` is the field mask.
+template <uint32_t Mask>
+__device__ __forceinline__ void write_record(uint8_t *rec, const Hit &h, const float o[3], const float d[3]) {
+#pragma clang fp contract(off)
+	constexpr Layout L = layout_of(Mask);
+	static_assert(L.size % 4 == 0 && L.size <= 32, "record size");
+	uint32_t w[L.size / 4];
+#pragma unroll
+	for (int i = 0; i < L.size / 4; ++i) w[i] = 0u;
+	const bool valid = h.t < __builtin_inff();
+	if constexpr ((Mask & F_UV) != 0) {
+		w[L.off[0] / 4] = __float_as_uint(valid ? h.u : 0.0f);
+		w[L.off[1] / 4] = __float_as_uint(valid ? h.v : 0.0f);
+	}
+	if constexpr ((Mask & F_T) != 0) w[L.off[2] / 4] = __float_as_uint(h.t);
+	if constexpr ((Mask & F_ID) != 0) w[L.off[3] / 4] = valid ? h.id : 0xFFFFFFFFu;
+	if constexpr ((Mask & F_VALID) != 0) w[L.off[4] / 4] |= (valid ? 1u : 0u) << (8 * (L.off[4] % 4));
+	if constexpr ((Mask & F_P) != 0) {
+		// bvh.hpp:261-263
+		w[L.off[5] / 4] = __float_as_uint(o[0] + h.t * d[0]);
+		w[L.off[6] / 4] = __float_as_uint(o[1] + h.t * d[1]);
+		w[L.off[7] / 4] = __float_as_uint(o[2] + h.t * d[2]);
+	}
+	if constexpr (L.size % 16 == 0) {
+#pragma unroll
+		for (int i = 0; i < L.size / 16; ++i)
+			reinterpret_cast<uint4 *>(rec)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+	} else if constexpr (L.size % 8 == 0) {
+#pragma unroll
+		for (int i = 0; i < L.size / 8; ++i) reinterpret_cast<uint2 *>(rec)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+	} else {
+#pragma unroll
+		for (int i = 0; i < L.size / 4; ++i) reinterpret_cast<uint32_t *>(rec)[i] = w[i];
+	}
+}
+
+// Dispatch on the runtime mask (uniform across the launch).
+__device__ __forceinline__ void write_any(uint32_t mask, uint8_t *rec, const Hit &h, const float o[3],
+                                          const float d[3]) {
+	switch (mask) {
+#define PRT_CASE(m)                                                                                \
+	case m: write_record<m>(rec, h, o, d); break;
+		PRT_CASE(0) PRT_CASE(1) PRT_CASE(2) PRT_CASE(3) PRT_CASE(4) PRT_CASE(5) PRT_CASE(6) PRT_CASE(7)
+		PRT_CASE(8) PRT_CASE(9) PRT_CASE(10) PRT_CASE(11) PRT_CASE(12) PRT_CASE(13) PRT_CASE(14)
+		PRT_CASE(15) PRT_CASE(16) PRT_CASE(17) PRT_CASE(18) PRT_CASE(19) PRT_CASE(20) PRT_CASE(21)
+		PRT_CASE(22) PRT_CASE(23) PRT_CASE(24) PRT_CASE(25) PRT_CASE(26) PRT_CASE(27) PRT_CASE(28)
+		PRT_CASE(29) PRT_CASE(30) PRT_CASE(31)
+#undef PRT_CASE
+	default: break;
+	}
+}
+
+}  // namespace prt

@@ -1,0 +1,103 @@
+// Host-side interfaces shared by the translation units of libprt_mi355x.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "common.h"
+
+namespace prt {
+
+// Device buffer owned by a context, grown on demand and reused across calls.
+struct DevBuf {
+	void *ptr = nullptr;
+	size_t bytes = 0;
+	hipError_t ensure(size_t want) {
+		if (want <= bytes) return hipSuccess;
+		if (ptr) {
+			hipError_t e = hipFree(ptr);
+			ptr = nullptr;
+			bytes = 0;
+			if (e != hipSuccess) return e;
+		}
+		if (want == 0) return hipSuccess;
+		hipError_t e = hipMalloc(&ptr, want);
+		if (e == hipSuccess) bytes = want;
+		return e;
+	}
+	void release() {
+		if (ptr) (void)hipFree(ptr);
+		ptr = nullptr;
+		bytes = 0;
+	}
+	template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// A BVH8 over one triangle set, resident in HBM.
+struct Bvh8 {
+	uint64_t n_tris = 0;
+	uint32_t n_nodes = 0;    // BVH8 nodes (both formats share topology)
+	uint32_t depth = 0;      // BVH8 levels
+	uint32_t leaf_size = 0;  // maximum triangles per leaf child
+	float scene_box[6] = {0, 0, 0, 0, 0, 0};  // min xyz, max xyz of all vertices
+	DevBuf nodes_q;          // NodeQ[n_nodes]
+	DevBuf nodes_f;          // NodeF[n_nodes]
+	DevBuf tris;             // TriRec[n_tris], in leaf order
+};
+
+// Scratch of the LBVH build, kept across set_tris calls.
+struct BuildScratch {
+	DevBuf tri_in;      // input triangles (9 floats each)
+	DevBuf tbox;        // float4[2*n] box per input triangle
+	DevBuf keys[2];     // uint64 Morton keys (ping-pong)
+	DevBuf vals[2];     // uint32 triangle indices (ping-pong)
+	DevBuf hist;        // radix histograms
+	DevBuf bin;         // binary tree: left, right, parent, first, count, flags
+	DevBuf bbox;        // float4[2*(2n-1)] binary node boxes
+	DevBuf bin_of;      // BVH8 node -> binary node
+	DevBuf tri_dest;    // sorted position -> TriRec slot
+	DevBuf misc;        // counters and scene bounds
+	void *host_misc = nullptr;  // pinned mirror of misc
+};
+
+struct BuildParams {
+	int leaf_size = 2;  // 1..kMaxLeaf
+};
+
+// Builds `out` from n triangles already in scratch.tri_in (device, 9 floats each).
+// Returns hipSuccess, or an error; *bad_input is set when a vertex is not finite.
+hipError_t build_bvh8(BuildScratch &s, uint64_t n, const BuildParams &p, Bvh8 &out, hipStream_t st,
+                      bool *bad_input, double *build_ms);
+
+// Traversal tunables.
+struct TraceParams {
+	int node_format = 1;      // 0 fp32 nodes, 1 8-bit quantized nodes
+	int cull = 1;             // 0 exact (no distance culling), 1 distance culling with margin
+	int block = 256;          // threads per workgroup
+	int waves_per_cu = 16;    // persistent waves per CU
+	int lds_stack = 8;        // LDS stack entries per thread (power of two)
+	int fetch_min = 16;       // idle lanes that trigger a batch fetch (64 = no compaction)
+	int steps = 8;            // traversal steps between fetches
+	int count_stats = 0;      // count node visits and triangle tests
+	float margin_rel = 0x1p-10f;
+	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)
+};
+
+struct TraceStats {
+	unsigned long long node_visits = 0, tri_tests = 0, leaf_visits = 0;
+};
+
+struct TraceScratch {
+	DevBuf counter;  // work counter + stats
+	DevBuf spill;    // per-thread stack spill
+	int n_cus = 0;
+};
+
+// Traces n device rays against `b`, writing HitReg records at `out` (device) with the layout
+// of `mask`. Launches on `st`; does not synchronise. `ms` receives the duration of the
+// traversal kernel measured with HIP events when non-null (synchronises then).
+hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats);
+
+}  // namespace prt

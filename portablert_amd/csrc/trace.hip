@@ -1,0 +1,450 @@
+// Nearest-hit traversal for nearest_hits (replaces BVH2::nearest_tri, bvh.hpp:224-265, and
+// the thread split of CPUBackend::nearest_hits, intersect_cpu.hpp:20-43).
+//
+// One lane per ray, persistent waves. A wave takes rays from a global counter in batches:
+// lanes whose ray has finished are counted with a ballot, one lane reserves that many ray
+// indices with a single atomic and the base is broadcast with readlane, so finished lanes are
+// refilled while the others keep traversing. The traversal stack lives in an LDS ring per lane
+// (spilling its oldest entries to global memory when deep). Entries carry the entry distance
+// of their box so they are re-culled when popped.
+//
+// Node tests are conservative versions of the reference slab test on the child boxes (fp32 or
+// 8-bit quantized boxes rounded outward), so a node is never rejected when the exact test of a
+// triangle below it would pass. Every triangle reached gets the exact reference predicates
+// (slab on its own box, then Moller-Trumbore, operation by operation). The answer is the
+// minimum t over all candidates with equal t resolved to the lowest primitive id.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "internal.h"
+
+namespace prt {
+namespace {
+
+struct KArgs {
+	const void *nodes;
+	const TriRec *tris;
+	const float *rays;
+	uint8_t *out;
+	uint32_t *counter;      // [0] work counter, [1..3] stats
+	uint2 *spill;           // spill_depth entries per launched thread
+	uint64_t n_rays;
+	uint32_t n_nodes;
+	uint32_t stride;
+	uint32_t mask;
+	uint32_t spill_depth;
+	uint32_t fetch_min;
+	uint32_t steps;
+	float margin_rel;
+	float margin_abs;
+	int cull;
+};
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+// Per-lane ray state.
+struct Ray {
+	float o[3], d[3], inv[3];
+	float tb, ub, vb;
+	uint32_t idb;
+	float abs_margin;  // margin_abs / |d|
+	bool degen;        // some 1/d is infinite: axis-parallel handling in node tests
+	bool patho;        // non-finite origin or NaN direction: every node passes
+};
+
+__device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float margin_abs, Ray &r) {
+	const float *p = rays + 6 * idx;  // 4-byte aligned AoS records of 24 bytes
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		r.o[k] = p[k];
+		r.d[k] = p[3 + k];
+	}
+#pragma unroll
+	for (int k = 0; k < 3; ++k) r.inv[k] = 1.0f / r.d[k];  // IEEE division, as the reference
+	r.degen = !(__builtin_isfinite(r.inv[0]) && __builtin_isfinite(r.inv[1]) && __builtin_isfinite(r.inv[2]));
+	r.patho = !(__builtin_isfinite(r.o[0]) && __builtin_isfinite(r.o[1]) && __builtin_isfinite(r.o[2])) ||
+	          __builtin_isnan(r.d[0]) || __builtin_isnan(r.d[1]) || __builtin_isnan(r.d[2]);
+	const float len = sqrtf(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
+	r.abs_margin = len > 0.0f ? margin_abs / len : __builtin_inff();
+	r.tb = __builtin_inff();
+	r.ub = r.vb = 0.0f;
+	r.idb = kInvalid;
+}
+
+// Distance beyond which no box can hold a better candidate (+inf: no culling).
+__device__ __forceinline__ float cull_limit(const Ray &r, const KArgs &a) {
+	if (!a.cull || !__builtin_isfinite(r.tb)) return __builtin_inff();
+	return r.tb + fmaxf(fabsf(r.tb) * a.margin_rel, r.abs_margin);
+}
+
+// Conservative slab test of one child box [lo, hi]. Returns the entry distance (tmin), or NaN
+// when the box is rejected.
+__device__ __forceinline__ float child_slab(const Ray &r, const float lo[3], const float hi[3], float limit) {
+#pragma clang fp contract(off)
+	float amin[3], amax[3];
+	bool poison = false;
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		const float t1 = (lo[k] - r.o[k]) * r.inv[k];
+		const float t2 = (hi[k] - r.o[k]) * r.inv[k];
+		amin[k] = fminf(t1, t2);
+		amax[k] = fmaxf(t1, t2);
+	}
+	if (r.degen) {
+		// 1/d infinite on an axis: a box side through the origin gives NaN in the exact test.
+		// Treat any box containing the origin's coordinate on that axis as unbounded there;
+		// on the x axis such a NaN makes the exact test pass outright (bvh.hpp:210-217 with
+		// std::max/std::min argument order), so the box must pass.
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			if (!__builtin_isfinite(r.inv[k]) && lo[k] <= r.o[k] && r.o[k] <= hi[k]) {
+				amin[k] = -__builtin_inff();
+				amax[k] = __builtin_inff();
+				if (k == 0) poison = true;
+			}
+		}
+	}
+	const float tmin = fmaxf(fmaxf(amin[0], amin[1]), amin[2]);
+	const float tmax = fminf(fminf(amax[0], amax[1]), amax[2]);
+	if (poison || r.patho) return -__builtin_inff();
+	const bool hit = (tmax >= 0.0f) && (tmin <= tmax) && (tmin <= limit);
+	return hit ? tmin : __builtin_nanf("");
+}
+
+// LDS ring stack: the newest `S` entries of each lane stay in LDS, older ones are moved to
+// the lane's spill region (entry at depth k lives at spill[k] once evicted).
+template <int S, int B>
+struct Stack {
+	uint2 *lds;    // [S][B]
+	uint2 *spill;  // this lane's spill region
+	uint32_t sp;
+	__device__ __forceinline__ void push(uint32_t ref, float tmin) {
+		uint2 *slot = lds + (sp & (S - 1)) * B;
+		if (sp >= static_cast<uint32_t>(S)) spill[sp - S] = *slot;
+		*slot = make_uint2(ref, __float_as_uint(tmin));
+		++sp;
+	}
+	__device__ __forceinline__ uint2 pop() {
+		--sp;
+		uint2 *slot = lds + (sp & (S - 1)) * B;
+		const uint2 e = *slot;
+		if (sp >= static_cast<uint32_t>(S)) *slot = spill[sp - S];
+		return e;
+	}
+};
+
+#define PRT_CE(i, j)                                                                               \
+	if (key[j] < key[i]) {                                                                         \
+		const float tk = key[i];                                                                   \
+		key[i] = key[j];                                                                           \
+		key[j] = tk;                                                                               \
+		const uint32_t tv = val[i];                                                                \
+		val[i] = val[j];                                                                           \
+		val[j] = tv;                                                                               \
+	}
+
+// Tests the children of one node; returns them sorted by entry distance in key/val
+// (val = kInvalid for misses, which sort last with key = +inf).
+template <class Node>
+__device__ __forceinline__ void test_children(const Node *nodes, uint32_t idx, const Ray &r, float limit,
+                                              float key[8], uint32_t val[8]);
+
+template <>
+__device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_t idx, const Ray &r, float limit,
+                                                     float key[8], uint32_t val[8]) {
+	const uint4 *src = reinterpret_cast<const uint4 *>(nodes + idx);
+	const uint4 w0 = src[0], w1 = src[1], w2 = src[2], w3 = src[3], w4 = src[4];
+	const float p[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+	const float s[3] = {exp2i(static_cast<int8_t>(w0.w & 0xFF)), exp2i(static_cast<int8_t>((w0.w >> 8) & 0xFF)),
+	                    exp2i(static_cast<int8_t>((w0.w >> 16) & 0xFF))};
+	const uint32_t child_base = w1.x, tri_base = w1.y;
+	const uint32_t meta_lo = w1.z, meta_hi = w1.w;
+	// qlo x: w2.x w2.y, qlo y: w2.z w2.w, qlo z: w3.x w3.y, qhi x: w3.z w3.w, qhi y: w4.x w4.y, qhi z: w4.z w4.w
+	const uint32_t ql[3][2] = {{w2.x, w2.y}, {w2.z, w2.w}, {w3.x, w3.y}};
+	const uint32_t qh[3][2] = {{w3.z, w3.w}, {w4.x, w4.y}, {w4.z, w4.w}};
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		const uint32_t meta = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+		float lo[3], hi[3];
+#pragma unroll
+		for (int ax = 0; ax < 3; ++ax) {
+			const uint32_t bl = (ql[ax][k >> 2] >> (8 * (k & 3))) & 0xFFu;
+			const uint32_t bh = (qh[ax][k >> 2] >> (8 * (k & 3))) & 0xFFu;
+			lo[ax] = fmaf(static_cast<float>(bl), s[ax], p[ax]);
+			hi[ax] = fmaf(static_cast<float>(bh), s[ax], p[ax]);
+		}
+		const float tmin = child_slab(r, lo, hi, limit);
+		const bool hit = meta != 0u && !__builtin_isnan(tmin);
+		key[k] = hit ? tmin : __builtin_inff();
+		val[k] = !hit ? kInvalid
+		              : (meta_internal(meta) ? child_base + meta_offset(meta)
+		                                     : leaf_entry(tri_base + meta_offset(meta), meta_count(meta)));
+	}
+	PRT_CE(0, 2) PRT_CE(1, 3) PRT_CE(4, 6) PRT_CE(5, 7) PRT_CE(0, 4) PRT_CE(1, 5) PRT_CE(2, 6) PRT_CE(3, 7)
+	PRT_CE(0, 1) PRT_CE(2, 3) PRT_CE(4, 5) PRT_CE(6, 7) PRT_CE(2, 4) PRT_CE(3, 5) PRT_CE(1, 4) PRT_CE(3, 6)
+	PRT_CE(1, 2) PRT_CE(3, 4) PRT_CE(5, 6)
+}
+
+template <>
+__device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_t idx, const Ray &r, float limit,
+                                                     float key[8], uint32_t val[8]) {
+	const NodeF *n = nodes + idx;
+	const uint4 h = *reinterpret_cast<const uint4 *>(n);
+	const uint32_t child_base = h.x, tri_base = h.y;
+	float lo[3][8], hi[3][8];
+#pragma unroll
+	for (int ax = 0; ax < 3; ++ax) {
+		const float4 *l4 = reinterpret_cast<const float4 *>(n->lo[ax]);
+		const float4 *h4 = reinterpret_cast<const float4 *>(n->hi[ax]);
+		const float4 a = l4[0], b = l4[1], c = h4[0], d = h4[1];
+		lo[ax][0] = a.x; lo[ax][1] = a.y; lo[ax][2] = a.z; lo[ax][3] = a.w;
+		lo[ax][4] = b.x; lo[ax][5] = b.y; lo[ax][6] = b.z; lo[ax][7] = b.w;
+		hi[ax][0] = c.x; hi[ax][1] = c.y; hi[ax][2] = c.z; hi[ax][3] = c.w;
+		hi[ax][4] = d.x; hi[ax][5] = d.y; hi[ax][6] = d.z; hi[ax][7] = d.w;
+	}
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		const uint32_t meta = ((k < 4 ? h.z : h.w) >> (8 * (k & 3))) & 0xFFu;
+		const float l[3] = {lo[0][k], lo[1][k], lo[2][k]};
+		const float u[3] = {hi[0][k], hi[1][k], hi[2][k]};
+		const float tmin = child_slab(r, l, u, limit);
+		const bool hit = meta != 0u && !__builtin_isnan(tmin);
+		key[k] = hit ? tmin : __builtin_inff();
+		val[k] = !hit ? kInvalid
+		              : (meta_internal(meta) ? child_base + meta_offset(meta)
+		                                     : leaf_entry(tri_base + meta_offset(meta), meta_count(meta)));
+	}
+	PRT_CE(0, 2) PRT_CE(1, 3) PRT_CE(4, 6) PRT_CE(5, 7) PRT_CE(0, 4) PRT_CE(1, 5) PRT_CE(2, 6) PRT_CE(3, 7)
+	PRT_CE(0, 1) PRT_CE(2, 3) PRT_CE(4, 5) PRT_CE(6, 7) PRT_CE(2, 4) PRT_CE(3, 5) PRT_CE(1, 4) PRT_CE(3, 6)
+	PRT_CE(1, 2) PRT_CE(3, 4) PRT_CE(5, 6)
+}
+#undef PRT_CE
+
+__device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ray &r) {
+	const uint32_t first = entry & kTriMask;
+	const uint32_t count = ((entry >> 29) & 3u) + 1u;
+	for (uint32_t m = 0; m < count; ++m) {
+		const uint4 *src = reinterpret_cast<const uint4 *>(tris + first + m);
+		const uint4 a = src[0], b = src[1], c = src[2];
+		const float v[9] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+		                    __uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y),
+		                    __uint_as_float(b.z), __uint_as_float(b.w), __uint_as_float(c.x)};
+		const uint32_t id = c.y;
+		float t, u, vv;
+		if (exact_candidate(v, r.o, r.d, r.inv, t, u, vv) && better(t, id, r.tb, r.idb)) {
+			r.tb = t;
+			r.ub = u;
+			r.vb = vv;
+			r.idb = id;
+		}
+	}
+}
+
+template <class Node, int S, int B, bool COUNT>
+__global__ __launch_bounds__(B) void k_trace(KArgs a) {
+	__shared__ uint2 lds[S * B];
+	const Node *nodes = static_cast<const Node *>(a.nodes);
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t lt_mask = lane ? (~0ull >> (64u - lane)) : 0ull;
+	Stack<S, B> st;
+	st.lds = lds + threadIdx.x;
+	st.spill = a.spill + static_cast<uint64_t>(blockIdx.x * B + threadIdx.x) * a.spill_depth;
+	st.sp = 0;
+
+	Ray r;
+	uint64_t ray_idx = ~0ull;
+	bool active = false;   // lane holds a ray
+	bool have_cur = false; // cur is the next entry to process
+	uint32_t cur = 0;
+	bool exhausted = false;  // wave-uniform: no rays left to fetch
+	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0;
+
+	while (true) {
+		// ---- refill finished lanes (wave-uniform control flow)
+		if (!exhausted) {
+			const uint64_t need = __ballot(!active);
+			const uint32_t n_need = __popcll(need);
+			const uint32_t n_act = 64u - n_need;
+			if (n_need >= a.fetch_min || n_act == 0u) {
+				const int leader = __ffsll(static_cast<unsigned long long>(need)) - 1;
+				uint64_t base = 0;
+				if (static_cast<int>(lane) == leader)
+					base = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter), static_cast<unsigned long long>(n_need));
+				base = __shfl(base, leader);
+				if (!active) {
+					const uint64_t idx = base + __popcll(need & lt_mask);
+					if (idx < a.n_rays) {
+						ray_idx = idx;
+						load_ray(a.rays, idx, a.margin_abs, r);
+						active = true;
+						have_cur = a.n_nodes > 0;
+						cur = 0;
+						st.sp = 0;
+					}
+				}
+				if (base + n_need >= a.n_rays) exhausted = true;
+			}
+		}
+		if (__ballot(active) == 0ull) break;
+
+		for (uint32_t step = 0; step < a.steps && active; ++step) {
+			if (!have_cur) {
+				const float limit = cull_limit(r, a);
+				while (st.sp > 0) {
+					const uint2 e = st.pop();
+					if (__uint_as_float(e.y) <= limit) {
+						cur = e.x;
+						have_cur = true;
+						break;
+					}
+				}
+			}
+			if (!have_cur) {
+				// ray finished
+				Hit h;
+				h.t = r.tb;
+				h.u = r.ub;
+				h.v = r.vb;
+				h.id = r.idb;
+				write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
+				active = false;
+				break;
+			}
+			if (cur & kLeafBit) {
+				test_leaf(a.tris, cur, r);
+				if (COUNT) {
+					++c_leaves;
+					c_tris += ((cur >> 29) & 3u) + 1u;
+				}
+				have_cur = false;
+			} else {
+				float key[8];
+				uint32_t val[8];
+				test_children<Node>(nodes, cur, r, cull_limit(r, a), key, val);
+				if (COUNT) ++c_nodes;
+				have_cur = val[0] != kInvalid;
+				cur = val[0];
+#pragma unroll
+				for (int k = 7; k >= 1; --k)
+					if (val[k] != kInvalid) st.push(val[k], key[k]);
+			}
+		}
+	}
+	if (COUNT) {
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 1, c_nodes);
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 2, c_tris);
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 3, c_leaves);
+	}
+}
+
+__global__ void k_zero(unsigned long long *p, int n) {
+	if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+
+template <class Node, int S, int B>
+hipError_t launch(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
+	if (count)
+		hipLaunchKernelGGL((k_trace<Node, S, B, true>), dim3(grid), dim3(B), 0, st, a);
+	else
+		hipLaunchKernelGGL((k_trace<Node, S, B, false>), dim3(grid), dim3(B), 0, st, a);
+	return hipGetLastError();
+}
+
+template <class Node, int B>
+hipError_t launch_s(const KArgs &a, uint32_t grid, int S, bool count, hipStream_t st) {
+	switch (S) {
+	case 4: return launch<Node, 4, B>(a, grid, count, st);
+	case 8: return launch<Node, 8, B>(a, grid, count, st);
+	case 16: return launch<Node, 16, B>(a, grid, count, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+template <class Node>
+hipError_t launch_b(const KArgs &a, uint32_t grid, int B, int S, bool count, hipStream_t st) {
+	switch (B) {
+	case 64: return launch_s<Node, 64>(a, grid, S, count, st);
+	case 128: return launch_s<Node, 128>(a, grid, S, count, st);
+	case 256: return launch_s<Node, 256>(a, grid, S, count, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+}  // namespace
+
+hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats) {
+#define PRT_TRY(x)                                                                                 \
+	do {                                                                                           \
+		hipError_t e_ = (x);                                                                       \
+		if (e_ != hipSuccess) return e_;                                                           \
+	} while (0)
+	if (n == 0) return hipSuccess;
+	if (s.n_cus == 0) {
+		int dev = 0;
+		PRT_TRY(hipGetDevice(&dev));
+		PRT_TRY(hipDeviceGetAttribute(&s.n_cus, hipDeviceAttributeMultiprocessorCount, dev));
+	}
+	const int B = p.block, S = p.lds_stack;
+	const uint64_t waves = static_cast<uint64_t>(s.n_cus) * static_cast<uint64_t>(std::max(1, p.waves_per_cu));
+	uint64_t grid64 = (waves * 64 + B - 1) / B;
+	const uint64_t need_threads = (n + 63) / 64 * 64;
+	grid64 = std::max<uint64_t>(1, std::min<uint64_t>(grid64, (need_threads + B - 1) / B));
+	const uint32_t grid = static_cast<uint32_t>(grid64);
+	// stack bound: each visited node pushes at most 7 entries, so 7 * depth + 1 suffice
+	const uint32_t max_entries = 7u * std::max(1u, b.depth) + 8u;
+	const uint32_t spill_depth = max_entries > static_cast<uint32_t>(S) ? max_entries - S : 1u;
+	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 4));
+	PRT_TRY(s.spill.ensure(sizeof(uint2) * static_cast<uint64_t>(spill_depth) * grid * B));
+
+	KArgs a;
+	a.nodes = p.node_format == 0 ? b.nodes_f.ptr : b.nodes_q.ptr;
+	a.tris = b.tris.as<TriRec>();
+	a.rays = rays;
+	a.out = static_cast<uint8_t *>(out);
+	a.counter = s.counter.as<uint32_t>();
+	a.spill = s.spill.as<uint2>();
+	a.n_rays = n;
+	a.n_nodes = b.n_nodes;
+	a.stride = static_cast<uint32_t>(layout_of(mask).size);
+	a.mask = mask;
+	a.spill_depth = spill_depth;
+	a.fetch_min = static_cast<uint32_t>(std::max(1, std::min(64, p.fetch_min)));
+	a.steps = static_cast<uint32_t>(std::max(1, p.steps));
+	a.margin_rel = p.margin_rel;
+	a.margin_abs = p.margin_abs;
+	a.cull = p.cull;
+
+	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 4);
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (ms) {
+		PRT_TRY(hipEventCreate(&e0));
+		PRT_TRY(hipEventCreate(&e1));
+		PRT_TRY(hipEventRecord(e0, st));
+	}
+	const bool count = p.count_stats != 0;
+	hipError_t e = p.node_format == 0 ? launch_b<NodeF>(a, grid, B, S, count, st)
+	                                  : launch_b<NodeQ>(a, grid, B, S, count, st);
+	PRT_TRY(e);
+	if (ms) {
+		PRT_TRY(hipEventRecord(e1, st));
+		PRT_TRY(hipEventSynchronize(e1));
+		PRT_TRY(hipEventElapsedTime(ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+	}
+	if (stats) {
+		unsigned long long h[4];
+		PRT_TRY(hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, st));
+		PRT_TRY(hipStreamSynchronize(st));
+		stats->node_visits = h[1];
+		stats->tri_tests = h[2];
+		stats->leaf_visits = h[3];
+	}
+	return hipSuccess;
+#undef PRT_TRY
+}
+
+}  // namespace prt

@@ -1,0 +1,110 @@
+"""Writes the golden vectors under tests/golden/ from the reference CPU backend.
+
+Runs the reference's own CPU backend (compiled from its sources into oracle/_ref by
+`make -C oracle`) on:
+  * the known-answer and edge-case scenes below (hand-made, tiny),
+  * samples of the deterministic workloads 0, 1 (primaries and bounces) and 2 at full scene
+    size (the scenes are regenerated from their seeds; only their SHA-256 is stored).
+Each fixture stores the rays (float32 x 6), the reference FullHitReg records (32 bytes each,
+t/u/v/p bit patterns, valid, and the id the reference's depth-first order picks) and the
+scene hash. Miss records keep the reference's indeterminate u/v/id bytes; tests ignore them.
+
+Usage: python tests/golden/make_golden.py   (needs /root/reference)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from portablert_amd import scenes  # noqa: E402
+
+
+def edge_scenes():
+    """Small scenes with the cases the reference's tests leave open."""
+    out = {}
+    kt = np.array([[-1, -1, 0, 1, -1, 0, 0, 1, 0]], np.float32)
+    # known answer (examples/validation/main.cpp:68-101)
+    out["known"] = (kt, np.array([[0.1, 0, -1, 0, 0, 1], [-2, 0, -1, 0, 0, 1]], np.float32))
+    # quad split on its diagonal, rays on the diagonal: equal-t ties (F2)
+    quad = np.array([[0, 0, 0, 1, 0, 0, 1, 1, 0], [0, 0, 0, 1, 1, 0, 0, 1, 0]], np.float32)
+    diag = np.array([[x, x, -1, 0, 0, 1] for x in (0.125, 0.25, 0.5, 0.75)] +
+                    [[x, x, 1, 0, 0, -1] for x in (0.25, 0.5)], np.float32)
+    out["tie"] = (quad, diag)
+    out["tie_swapped"] = (quad[::-1].copy(), diag)
+    # triangle behind the origin, origin inside its box: negative t (F1)
+    tilt = np.array([[-1, -1, -0.5, 1, -1, 0.5, 0, 1, 0.0]], np.float32)
+    neg = np.array([[0.0, -0.2, 0.1, 0, 0, 1], [0.2, 0.0, 0.3, 0, 0, 1], [0.0, 0.0, 2.0, 0, 0, 1],
+                    [0.1, -0.1, 0.05, 0.3, 0.2, 0.9]], np.float32)
+    out["negative_t"] = (tilt, neg)
+    # axis-parallel rays, zero direction components, rays in the triangle plane (det == 0)
+    ax = np.array([[0.1, 0, -1, 0, 0, 1], [0.1, 0, 1, 0, 0, -1], [-1, -1, -1, 0, 0, 1],
+                   [0, 0, -1, 0, 0, 1], [-5, 0, 0, 1, 0, 0], [0, -5, 0, 0, 1, 0],
+                   [1, -1, -1, 0, 0, 1], [0.5, 0.0, 0.0, 0, 0, 1], [0, 0, 0, 0, 0, 0],
+                   [0.3, 0.2, -2, -0.0, 0.0, 1]], np.float32)
+    out["axis_parallel"] = (kt, ax)
+    # two coplanar copies and a stack of parallel triangles (ties across ids, nearest of many)
+    stack = np.concatenate([kt + np.array([0, 0, z, 0, 0, z, 0, 0, z], np.float32) for z in
+                            (0.5, 0.0, 0.25, 0.0, -0.25, 0.5)])
+    sr = np.array([[0.1, 0.0, -1, 0, 0, 1], [0.1, 0.0, 1, 0, 0, -1], [0.0, 0.0, 0.1, 0, 0, 1],
+                   [0.0, 0.0, 0.1, 0, 0, -1], [0.05, -0.5, -2, 0.01, 0.02, 1]], np.float32)
+    out["stack"] = (stack, sr)
+    # single triangle (bvh.hpp:165-181) and rays missing it
+    out["single"] = (kt, np.array([[0, 0, -1, 0.1, 0.1, 1], [5, 5, -1, 0, 0, 1], [0, 0, 1, 0, 0, 1]], np.float32))
+    # random small soups with random rays, including origins inside boxes
+    rng = np.random.default_rng(12345)
+    for k, n in enumerate((2, 3, 7, 33, 200)):
+        c = rng.uniform(-1, 1, (n, 1, 3))
+        tris = (c + rng.uniform(-0.3, 0.3, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+        o = rng.uniform(-1.2, 1.2, (256, 3))
+        d = rng.normal(size=(256, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        out[f"soup{n}"] = (tris, np.concatenate([o, d], 1).astype(np.float32))
+    return out
+
+
+def workload_samples():
+    rng = np.random.default_rng(777)
+    res = {}
+    for w, count in ((0, 4096), (2, 4096)):
+        tris = scenes.scene(w)
+        total = scenes.n_rays(w)
+        idx = np.sort(rng.choice(total, count, replace=False))
+        res[f"workload{w}"] = (tris, scenes.ray_sample(w, idx), idx)
+    # workload 1: primaries and the bounce rays they spawn (traced with the reference)
+    tris = scenes.scene(1)
+    total = scenes.n_rays(1)
+    idx = np.sort(rng.choice(total, 4096, replace=False))
+    prim = scenes.ray_sample(1, idx)
+    res["workload1_primary"] = (tris, prim, idx)
+    hits = oracle.ref_trace(tris, prim)
+    p = np.stack([hits["px"], hits["py"], hits["pz"]], 1)
+    bounce = scenes.bounce_rays(tris, prim, hits["t"], hits["primitive_id"], p)
+    res["workload1_bounce"] = (tris, bounce[:4096], None)
+    return res
+
+
+def main():
+    if not oracle.have_ref():
+        raise SystemExit("build the reference shim first: make -C oracle")
+    for name, (tris, rays) in edge_scenes().items():
+        hits = oracle.ref_trace(tris, rays)
+        np.savez_compressed(os.path.join(HERE, f"edge_{name}.npz"), tris=tris, rays=rays,
+                            hits=hits.view(np.uint8).reshape(-1, 32))
+    for name, (tris, rays, idx) in workload_samples().items():
+        hits = oracle.ref_trace(tris, rays)
+        extra = {} if idx is None else {"index": idx.astype(np.uint64)}
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), scene_sha256=np.array(scenes.sha256(tris)),
+                            rays=rays, hits=hits.view(np.uint8).reshape(-1, 32), **extra)
+    # HitReg layouts of the reference headers for all 32 field masks
+    lay = np.stack([oracle.ref_layout(m) if m else np.array([8] + [-1] * 8, np.int32) for m in range(32)])
+    np.save(os.path.join(HERE, "hitreg_layouts.npy"), lay)
+    print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith((".npz", ".npy"))))
+
+
+if __name__ == "__main__":
+    main()

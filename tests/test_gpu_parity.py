@@ -1,0 +1,247 @@
+"""Parity of the gfx950 nearest-hit path with the CPU oracle, through the C ABI.
+
+The bar is bit-exact: t, u, v, p bit patterns, valid and primitive_id (lowest id at equal t)
+equal to the oracle's lowest-id mode; against the reference's own golden records (depth-first
+tie order) every field but primitive_id is bit-exact and primitive_id is equal wherever the
+reference result has no equal-t tie. u, v and primitive_id are compared only on hits (the
+reference leaves them indeterminate on misses).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import portablert_amd as prt
+from portablert_amd import scenes
+from conftest import GOLDEN, load_golden
+from helpers import as_full, assert_same
+
+pytestmark = pytest.mark.gpu
+
+FULL = prt.mask_of(prt.ALL_TAGS)
+VARIANTS = [(fmt, cull) for fmt in (1, 0) for cull in (1, 0)]  # (node format, culling)
+EDGE = sorted(f[5:-4] for f in os.listdir(GOLDEN) if f.startswith("edge_"))
+
+
+def configure(ctx, fmt=1, cull=1):
+    ctx.set_option("node_format", fmt)
+    ctx.set_option("cull", cull)
+
+
+def trace(ctx, tris, rays, mask=FULL, variants=VARIANTS, leaf_size=None):
+    """Runs every variant; asserts they agree bit for bit; returns the first result."""
+    if leaf_size is not None:
+        ctx.set_option("leaf_size", leaf_size)
+    ctx.set_tris(tris)
+    first = None
+    for fmt, cull in variants:
+        configure(ctx, fmt, cull)
+        got = ctx.nearest_hits(rays, mask)
+        if first is None:
+            first = got
+        else:
+            assert first.tobytes() == got.tobytes(), f"variant fmt={fmt} cull={cull} differs"
+    configure(ctx)
+    return first
+
+
+def test_known_answer(gpu_ctx):
+    """examples/validation/main.cpp:66-111 with its own tolerance (1e-4) and exactly."""
+    tri = np.array([[-1, -1, 0, 1, -1, 0, 0, 1, 0]], np.float32)
+    hit_ray = np.array([[0.1, 0, -1, 0, 0, 1]], np.float32)
+    miss_ray = np.array([[-2, 0, -1, 0, 0, 1]], np.float32)
+    gpu_ctx.set_tris(tri)
+    h = gpu_ctx.nearest_hits(hit_ray)[0]
+    assert h["valid"] == 1 and h["primitive_id"] == 0
+    for f, want in (("t", 1.0), ("u", 0.3), ("v", 0.5), ("px", 0.1), ("py", 0.0), ("pz", 0.0)):
+        assert abs(float(h[f]) - want) <= 1e-4, (f, h[f])
+    m = gpu_ctx.nearest_hits(miss_ray)[0]
+    assert m["valid"] == 0 and np.isinf(m["t"])
+    # filtered record HitReg<valid>
+    assert gpu_ctx.nearest_hits(hit_ray, prt.VALID)["valid"][0] == 1
+    assert gpu_ctx.nearest_hits(miss_ray, prt.VALID)["valid"][0] == 0
+    # exact agreement with the oracle on both
+    both = np.concatenate([hit_ray, miss_ray])
+    assert_same(gpu_ctx.nearest_hits(both), oracle.trace(tri, both), what="known")
+
+
+@pytest.mark.parametrize("name", EDGE)
+def test_edge_fixtures(gpu_ctx, name):
+    g = load_golden("edge_" + name)
+    tris, rays, ref = g["tris"], g["rays"], as_full(g["hits"])
+    print(f"edge {name}: {len(tris)} tris {len(rays)} rays", flush=True)
+    for leaf in (1, 2, 4):
+        got = trace(gpu_ctx, tris, rays, leaf_size=leaf)
+        want = oracle.trace(tris, rays, oracle.LOWEST_ID)
+        assert_same(got, want, what=f"{name} leaf={leaf} vs oracle")
+        assert_same(got, ref, ids=False, what=f"{name} vs reference")
+    gpu_ctx.set_option("leaf_size", 2)
+
+
+@pytest.mark.parametrize("name", ["workload0", "workload1_primary", "workload1_bounce", "workload2"])
+def test_workload_fixtures(gpu_ctx, name):
+    g = load_golden(name)
+    w = int(name[8])
+    tris = scenes.scene(w)
+    assert scenes.sha256(tris) == str(g["scene_sha256"]), "scene generator is not deterministic"
+    rays, ref = g["rays"], as_full(g["hits"])
+    print(f"{name}: {len(tris)} tris {len(rays)} rays", flush=True)
+    got = trace(gpu_ctx, tris, rays)
+    want = oracle.trace(tris, rays, oracle.LOWEST_ID)
+    assert_same(got, want, what=f"{name} vs oracle lowest-id")
+    assert_same(got, ref, ids=False, what=f"{name} vs reference records")
+    tie_free = oracle.trace(tris, rays, oracle.DFS)["primitive_id"] == want["primitive_id"]
+    valid = ref["t"] < np.inf
+    assert (got["primitive_id"][valid & tie_free] == ref["primitive_id"][valid & tie_free]).all()
+
+
+def test_workload0_full(gpu_ctx):
+    """All 1,048,576 primary rays of the 16,384-triangle soup, t + primitive_id."""
+    tris, rays = scenes.scene(0), scenes.rays(0)
+    mask = prt.T | prt.PRIMITIVE_ID
+    got = trace(gpu_ctx, tris, rays, mask)
+    want = oracle.trace(tris, rays, oracle.LOWEST_ID)
+    assert_same(got, want, fields=("t", "primitive_id"), what="workload0 full")
+    print(f"hits {int((got['t'] < np.inf).sum())}", flush=True)
+
+
+def test_workload2_full_variants_and_sample(gpu_ctx):
+    """8,388,608 incoherent rays in the 262,144-triangle room: all four variants identical,
+    and a 65,536-ray random sample equal to the oracle."""
+    tris, rays = scenes.scene(2), scenes.rays(2)
+    mask = prt.T | prt.PRIMITIVE_ID
+    got = trace(gpu_ctx, tris, rays, mask)
+    idx = np.random.default_rng(5).choice(len(rays), 65536, replace=False)
+    want = oracle.trace(tris, rays[idx], oracle.LOWEST_ID)
+    assert_same(got[idx], want, fields=("t", "primitive_id"), what="workload2 sample")
+    neg = float((got["t"] < 0).mean())
+    print(f"hit rate {float((got['t'] < np.inf).mean()):.4f} negative-t rate {neg:.4f}", flush=True)
+
+
+def test_all_field_masks(gpu_ctx):
+    """Every HitReg<Tags...> layout holds the same values as the full record."""
+    tris = scenes.scene(0)
+    rays = scenes.rays(0, first=400000, count=3000)
+    gpu_ctx.set_tris(tris)
+    full = gpu_ctx.nearest_hits(rays, FULL)
+    for mask in range(32):
+        got = gpu_ctx.nearest_hits(rays, mask)
+        size, off = prt.layout(mask)
+        assert got.dtype.itemsize == size
+        raw = got.view(np.uint8).reshape(len(rays), size)
+        covered = np.zeros(size, bool)
+        for name in got.dtype.names:
+            a, b = got[name], full[name]
+            if name in ("u", "v", "primitive_id"):
+                hit = full["t"] < np.inf
+                a, b = a[hit], b[hit]
+            assert a.tobytes() == b.tobytes(), (mask, name)
+            o = got.dtype.fields[name][1]
+            covered[o:o + got.dtype.fields[name][0].itemsize] = True
+        assert (raw[:, ~covered] == 0).all(), f"padding bytes of mask {mask} not zero"
+
+
+def test_device_entry_point_matches_host(gpu_ctx):
+    import torch
+    tris = scenes.scene(0)
+    rays = scenes.rays(0, first=300000, count=70001)
+    mask = prt.T | prt.PRIMITIVE_ID | prt.UV
+    gpu_ctx.set_tris(tris)
+    host = gpu_ctx.nearest_hits(rays, mask)
+    size, _ = prt.layout(mask)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_out = torch.zeros(len(rays) * size, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    gpu_ctx.nearest_hits_device(d_rays.data_ptr(), len(rays), mask, d_out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    dev = d_out.cpu().numpy().view(prt.hit_dtype(mask))
+    assert dev.tobytes() == host.tobytes()
+
+
+def test_sizes_and_empty(gpu_ctx):
+    tris = scenes.scene(0)[:1000]
+    gpu_ctx.set_tris(tris)
+    for n in (1, 2, 63, 64, 65, 1000, 4097):
+        rays = scenes.rays(0, first=500000, count=n)
+        assert_same(gpu_ctx.nearest_hits(rays), oracle.trace(tris, rays), what=f"n={n}")
+    assert len(gpu_ctx.nearest_hits(np.zeros((0, 6), np.float32))) == 0
+    # empty scene: every ray misses
+    gpu_ctx.set_tris(np.zeros((0, 9), np.float32))
+    out = gpu_ctx.nearest_hits(scenes.rays(0, first=500000, count=100))
+    assert (out["valid"] == 0).all() and np.isinf(out["t"]).all()
+    assert (out["primitive_id"] == 0xFFFFFFFF).all()
+
+
+def test_error_paths(gpu_ctx):
+    fresh = prt.Context(0)
+    rays = scenes.rays(0, first=0, count=10)
+    with pytest.raises(prt.PrtError) as e:
+        fresh.nearest_hits(rays)
+    assert e.value.code == prt.ERR_STATE
+    bad = scenes.scene(0)[:10].copy()
+    bad[3, 4] = np.nan
+    with pytest.raises(prt.PrtError) as e:
+        fresh.set_tris(bad)
+    assert e.value.code == prt.ERR_INPUT
+    fresh.set_option("alloc_limit", 1000)
+    with pytest.raises(prt.PrtError) as e:
+        fresh.set_tris(scenes.scene(0))
+    assert e.value.code == prt.ERR_OOM
+    fresh.set_option("alloc_limit", 0)
+    fresh.set_tris(scenes.scene(0)[:100])
+    # layout mismatch through the raw entry point
+    import ctypes
+    out = np.zeros(10 * 32, np.uint8)
+    offs = np.zeros(8, np.int32)
+    rc = prt.lib().prt_mi355x_nearest_hits(fresh.h, rays.ctypes.data_as(ctypes.c_void_p), 10, 31,
+                                            out.ctypes.data_as(ctypes.c_void_p), 32,
+                                            offs.ctypes.data_as(ctypes.c_void_p))
+    assert rc == prt.ERR_LAYOUT
+    # still usable after errors
+    assert_same(fresh.nearest_hits(rays), oracle.trace(scenes.scene(0)[:100], rays), what="after errors")
+    fresh.close()
+
+
+def test_bvh_invariants(gpu_ctx):
+    """Exported BVH: every triangle once; quantized child boxes contain the fp32 ones; fp32
+    child boxes contain their triangles."""
+    tris = scenes.scene(2)
+    gpu_ctx.set_tris(tris)
+    nq = gpu_ctx.export("nodes_q")
+    nf = gpu_ctx.export("nodes_f")
+    tr = gpu_ctx.export("tris").view(np.uint32).reshape(-1, 12)
+    ids = tr[:, 9]
+    assert np.array_equal(np.sort(ids), np.arange(len(tris), dtype=np.uint32))
+    assert np.array_equal(tr[:, :9].view(np.float32), tris[ids])
+    n_nodes = len(nf) // 208
+    assert len(nq) // 80 == n_nodes
+    f = nf.reshape(n_nodes, 208)
+    meta = f[:, 8:16]
+    lo = f[:, 16:112].copy().view(np.float32).reshape(n_nodes, 3, 8)
+    hi = f[:, 112:208].copy().view(np.float32).reshape(n_nodes, 3, 8)
+    q = nq.reshape(n_nodes, 80)
+    p = q[:, 0:12].copy().view(np.float32)
+    e = q[:, 12:15].view(np.int8).astype(np.int32)
+    scale = (2.0 ** e).astype(np.float32)
+    qlo = q[:, 32:56].reshape(n_nodes, 3, 8).astype(np.float32)
+    qhi = q[:, 56:80].reshape(n_nodes, 3, 8).astype(np.float32)
+    used = meta != 0
+    # fp32 model of fmaf(q, 2^e, p): the product is exact, so one rounding of the sum
+    dlo = (qlo * scale[:, :, None]).astype(np.float64) + p[:, :, None].astype(np.float64)
+    dhi = (qhi * scale[:, :, None]).astype(np.float64) + p[:, :, None].astype(np.float64)
+    dlo, dhi = dlo.astype(np.float32), dhi.astype(np.float32)
+    u3 = np.repeat(used[:, None, :], 3, 1)
+    assert (dlo[u3] <= lo[u3]).all() and (dhi[u3] >= hi[u3]).all()
+    # leaves: boxes contain their triangles
+    tbase = f[:, 4:8].copy().view(np.uint32)[:, 0]
+    v = tr[:, :9].view(np.float32).reshape(-1, 3, 3)
+    for k in range(8):
+        m = meta[:, k]
+        leaf = (m != 0) & ((m & 0xE0) != 0xE0)
+        cnt, off = m >> 5, m & 0x1F
+        for node in np.flatnonzero(leaf)[:5000]:
+            s = int(tbase[node]) + int(off[node])
+            tv = v[s:s + int(cnt[node])]
+            assert (tv.min(axis=(0, 1)) >= lo[node, :, k]).all()
+            assert (tv.max(axis=(0, 1)) <= hi[node, :, k]).all()
