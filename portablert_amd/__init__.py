@@ -41,6 +41,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is not built: run python portablert_amd/build.py")
+        # PyTorch ships its own HIP runtime under the same SONAME (libamdhip64.so.7) but links it
+        # by file name, so loading this library first would put two HIP runtimes in the process
+        # and torch would then see no GPU. Loading torch first makes this library bind to the
+        # runtime torch already loaded: one runtime per process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB)
         vp, u64, ci, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32
         L.prt_mi355x_probe.argtypes = [ctypes.POINTER(ci), ctypes.c_char_p, ctypes.c_size_t]

@@ -160,6 +160,7 @@ void prt_mi355x_destroy(prt_ctx *c) {
 	                       &c->out_stage};
 	for (auto *b : bufs) b->release();
 	if (c->bs.host_misc) (void)hipHostFree(c->bs.host_misc);
+	if (c->ts.host_err) (void)hipHostFree(c->ts.host_err);
 	(void)hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -233,6 +234,9 @@ int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t 
 	e = hipMemcpyAsync(out, c->out_stage.ptr, n * static_cast<uint64_t>(stride), hipMemcpyDeviceToHost, c->stream);
 	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: result copy");
+	if (c->ts.host_err && *c->ts.host_err)
+		return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " +
+		                                std::to_string(*c->ts.host_err) + ")");
 	c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	return PRT_OK;
 }
@@ -255,7 +259,10 @@ int prt_mi355x_nearest_hits_device(prt_ctx *c, const float *d_rays, uint64_t n, 
 int prt_mi355x_synchronize(prt_ctx *c) {
 	if (!c) return PRT_ERR_ARG;
 	hipError_t e = hipStreamSynchronize(c->stream);
-	return e == hipSuccess ? PRT_OK : hip_fail(c, e, "synchronize");
+	if (e != hipSuccess) return hip_fail(c, e, "synchronize");
+	if (c->ts.host_err && *c->ts.host_err)
+		return fail(c, PRT_ERR_HIP, "traversal found a corrupt BVH entry (flags " + std::to_string(*c->ts.host_err) + ")");
+	return PRT_OK;
 }
 
 int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {

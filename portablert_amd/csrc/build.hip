@@ -298,7 +298,8 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	t.first[node] = j;
 	__threadfence();
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
-	while (p != kNone) {
+	// a radix tree over at most 2^29 keys is at most 64 + 29 levels deep
+	for (int guard = 0; p != kNone && guard < 128; ++guard) {
 		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
 		__threadfence();
 		const uint32_t a = t.left[p], b = t.right[p];
@@ -493,16 +494,25 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 
 __global__ __launch_bounds__(kBlock) void k_emit_tris(const float *__restrict__ tris, const uint32_t *__restrict__ sorted,
                                                       const uint32_t *__restrict__ dest, uint32_t n,
-                                                      TriRec *__restrict__ out) {
+                                                      TriRec *__restrict__ out, uint32_t *misc) {
 	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
 	if (j >= n) return;
 	const uint32_t src = sorted[j];
+	if (dest[j] >= n || src >= n) {  // a triangle not placed by the collapse: report, never write
+		atomicOr(misc + M_ERR, 2u);
+		return;
+	}
 	TriRec r;
 #pragma unroll
 	for (int k = 0; k < 9; ++k) r.v[k] = tris[9ull * src + k];
 	r.id = src;
 	r.pad[0] = r.pad[1] = 0;
 	out[dest[j]] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t *p, uint32_t n, uint32_t v) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) p[i] = v;
 }
 
 __global__ void k_root(uint32_t *misc, uint32_t *bin_of, uint32_t root) {
@@ -597,6 +607,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), t);
 	hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
 	                   s.vals[cur].as<uint32_t>(), t);
+	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
 	// BVH8 levels; the root is binary node 0 (or leaf 0 when n == 1)
 	hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, misc, s.bin_of.as<uint32_t>(), 0u);
 	CollapseArgs ca;
@@ -620,7 +631,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		}
 	}
 	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(),
-	                   s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>());
+	                   s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>(), misc);
 	PRT_TRY(hipGetLastError());
 	PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * M_WORDS, hipMemcpyDeviceToHost, st));
 	if (build_ms) PRT_TRY(hipEventRecord(ev1, st));
@@ -632,7 +643,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		(void)hipEventDestroy(ev0);
 		(void)hipEventDestroy(ev1);
 	}
-	*bad_input = hm[M_ERR] != 0;
+	*bad_input = (hm[M_ERR] & 1u) != 0;
+	if (hm[M_ERR] & 2u) return hipErrorUnknown;  // inconsistent build
 	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SB + k]);
 	out.n_nodes = hm[M_NODES];
 	out.depth = hm[M_DEPTH];

@@ -86,12 +86,14 @@ struct TraceParams {
 
 struct TraceStats {
 	unsigned long long node_visits = 0, tri_tests = 0, leaf_visits = 0;
+	uint32_t errors = 0;  // bit 0 corrupt leaf entry, bit 1 corrupt node / stack bound
 };
 
 struct TraceScratch {
 	DevBuf counter;  // work counter + stats
 	DevBuf spill;    // per-thread stack spill
 	int n_cus = 0;
+	uint32_t *host_err = nullptr;  // pinned copy of the last launch's error flags
 };
 
 // Traces n device rays against `b`, writing HitReg records at `out` (device) with the layout

@@ -32,6 +32,8 @@ struct KArgs {
 	uint2 *spill;           // spill_depth entries per launched thread
 	uint64_t n_rays;
 	uint32_t n_nodes;
+	uint32_t n_tris;
+	uint32_t max_entries;   // stack bound 7 * depth + 8
 	uint32_t stride;
 	uint32_t mask;
 	uint32_t spill_depth;
@@ -313,6 +315,11 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				break;
 			}
 			if (cur & kLeafBit) {
+				if ((cur & kTriMask) + ((cur >> 29) & 3u) + 1u > a.n_tris) {  // corrupt entry: report
+					atomicOr(a.counter + 8, 1u);
+					have_cur = false;
+					continue;
+				}
 				test_leaf(a.tris, cur, r);
 				if (COUNT) {
 					++c_leaves;
@@ -320,6 +327,11 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				}
 				have_cur = false;
 			} else {
+				if (cur >= a.n_nodes || st.sp + 7u > a.max_entries) {  // corrupt node or stack bound
+					atomicOr(a.counter + 8, 2u);
+					have_cur = false;
+					continue;
+				}
 				float key[8];
 				uint32_t val[8];
 				test_children<Node>(nodes, cur, r, cull_limit(r, a), key, val);
@@ -396,7 +408,7 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	// stack bound: each visited node pushes at most 7 entries, so 7 * depth + 1 suffice
 	const uint32_t max_entries = 7u * std::max(1u, b.depth) + 8u;
 	const uint32_t spill_depth = max_entries > static_cast<uint32_t>(S) ? max_entries - S : 1u;
-	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 4));
+	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 5));
 	PRT_TRY(s.spill.ensure(sizeof(uint2) * static_cast<uint64_t>(spill_depth) * grid * B));
 
 	KArgs a;
@@ -408,6 +420,8 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.spill = s.spill.as<uint2>();
 	a.n_rays = n;
 	a.n_nodes = b.n_nodes;
+	a.n_tris = static_cast<uint32_t>(b.n_tris);
+	a.max_entries = max_entries;
 	a.stride = static_cast<uint32_t>(layout_of(mask).size);
 	a.mask = mask;
 	a.spill_depth = spill_depth;
@@ -417,7 +431,7 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.margin_abs = p.margin_abs;
 	a.cull = p.cull;
 
-	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 4);
+	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 5);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (ms) {
 		PRT_TRY(hipEventCreate(&e0));
@@ -428,6 +442,11 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	hipError_t e = p.node_format == 0 ? launch_b<NodeF>(a, grid, B, S, count, st)
 	                                  : launch_b<NodeQ>(a, grid, B, S, count, st);
 	PRT_TRY(e);
+	if (!s.host_err) {
+		PRT_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.host_err), sizeof(uint32_t)));
+		*s.host_err = 0;
+	}
+	PRT_TRY(hipMemcpyAsync(s.host_err, a.counter + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 	if (ms) {
 		PRT_TRY(hipEventRecord(e1, st));
 		PRT_TRY(hipEventSynchronize(e1));
@@ -436,12 +455,13 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 		(void)hipEventDestroy(e1);
 	}
 	if (stats) {
-		unsigned long long h[4];
+		unsigned long long h[5];
 		PRT_TRY(hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, st));
 		PRT_TRY(hipStreamSynchronize(st));
 		stats->node_visits = h[1];
 		stats->tri_tests = h[2];
 		stats->leaf_visits = h[3];
+		stats->errors = static_cast<uint32_t>(h[4]);
 	}
 	return hipSuccess;
 #undef PRT_TRY

@@ -19,8 +19,13 @@ def as_full(raw):
 
 def diff_records(got, want, fields=FIELDS, ids=True):
     """Dict field -> number of rays whose bits differ. u, v and primitive_id are compared only
-    where want is a hit (the reference leaves them indeterminate on a miss)."""
+    where want is a hit (the reference leaves them indeterminate on a miss). With ids=False
+    (comparing against the reference's depth-first tie order) u and v are compared only where
+    both picked the same triangle; t and p must match everywhere."""
     want_valid = np.asarray(want["t"]) < np.inf
+    same_tri = np.ones(len(want), bool)
+    if not ids and "primitive_id" in got.dtype.names and "primitive_id" in want.dtype.names:
+        same_tri = np.asarray(got["primitive_id"]) == np.asarray(want["primitive_id"])
     out = {}
     for f in fields:
         if f not in got.dtype.names or f not in want.dtype.names:
@@ -32,7 +37,7 @@ def diff_records(got, want, fields=FIELDS, ids=True):
             g, w = g != 0, w != 0
         bad = g != w
         if f in ("u", "v", "primitive_id"):
-            bad &= want_valid
+            bad &= want_valid & same_tri
         n = int(bad.sum())
         if n:
             out[f] = n
