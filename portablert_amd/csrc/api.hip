@@ -232,11 +232,11 @@ int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t 
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray copy");
 	if (int r = trace_common(c, c->rays_stage.as<float>(), n, mask, c->out_stage.ptr, c->stream)) return r;
 	e = hipMemcpyAsync(out, c->out_stage.ptr, n * static_cast<uint64_t>(stride), hipMemcpyDeviceToHost, c->stream);
-	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+	uint32_t flags = 0;
+	if (e == hipSuccess) e = prt::trace_errors(c->ts, c->stream, &flags);
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: result copy");
-	if (c->ts.host_err && *c->ts.host_err)
-		return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " +
-		                                std::to_string(*c->ts.host_err) + ")");
+	if (flags)
+		return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
 	c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	return PRT_OK;
 }
@@ -258,10 +258,11 @@ int prt_mi355x_nearest_hits_device(prt_ctx *c, const float *d_rays, uint64_t n, 
 
 int prt_mi355x_synchronize(prt_ctx *c) {
 	if (!c) return PRT_ERR_ARG;
-	hipError_t e = hipStreamSynchronize(c->stream);
+	hipError_t e = hipDeviceSynchronize();  // device entry points may use caller streams
+	uint32_t flags = 0;
+	if (e == hipSuccess) e = prt::trace_errors(c->ts, c->stream, &flags);
 	if (e != hipSuccess) return hip_fail(c, e, "synchronize");
-	if (c->ts.host_err && *c->ts.host_err)
-		return fail(c, PRT_ERR_HIP, "traversal found a corrupt BVH entry (flags " + std::to_string(*c->ts.host_err) + ")");
+	if (flags) return fail(c, PRT_ERR_HIP, "traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
 	return PRT_OK;
 }
 
@@ -293,13 +294,13 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		if (v != 4 && v != 8 && v != 16) return bad();
 		c->tp.lds_stack = static_cast<int>(v);
 		break;
-	case PRT_OPT_FETCH_MIN:
-		if (v < 1 || v > 64) return bad();
-		c->tp.fetch_min = static_cast<int>(v);
+	case PRT_OPT_CHUNK:
+		if (v < 64 || v > 65536) return bad();
+		c->tp.chunk = static_cast<int>(v);
 		break;
-	case PRT_OPT_STEPS:
-		if (v < 1 || v > 4096) return bad();
-		c->tp.steps = static_cast<int>(v);
+	case PRT_OPT_LEAF_PHASE:
+		if (v < 1 || v > 64) return bad();
+		c->tp.leaf_phase = static_cast<int>(v);
 		break;
 	case PRT_OPT_COUNT_STATS: c->tp.count_stats = v ? 1 : 0; break;
 	case PRT_OPT_TIME_KERNEL: c->time_kernel = v ? 1 : 0; break;
@@ -326,8 +327,8 @@ int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
 	case PRT_OPT_BLOCK: *v = c->tp.block; break;
 	case PRT_OPT_WAVES_PER_CU: *v = c->tp.waves_per_cu; break;
 	case PRT_OPT_LDS_STACK: *v = c->tp.lds_stack; break;
-	case PRT_OPT_FETCH_MIN: *v = c->tp.fetch_min; break;
-	case PRT_OPT_STEPS: *v = c->tp.steps; break;
+	case PRT_OPT_CHUNK: *v = c->tp.chunk; break;
+	case PRT_OPT_LEAF_PHASE: *v = c->tp.leaf_phase; break;
 	case PRT_OPT_COUNT_STATS: *v = c->tp.count_stats; break;
 	case PRT_OPT_TIME_KERNEL: *v = c->time_kernel; break;
 	case PRT_OPT_ALLOC_LIMIT: *v = c->alloc_limit; break;

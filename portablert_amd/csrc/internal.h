@@ -75,10 +75,10 @@ struct TraceParams {
 	int node_format = 1;      // 0 fp32 nodes, 1 8-bit quantized nodes
 	int cull = 1;             // 0 exact (no distance culling), 1 distance culling with margin
 	int block = 256;          // threads per workgroup
-	int waves_per_cu = 16;    // persistent waves per CU
+	int waves_per_cu = 32;    // persistent waves per CU
 	int lds_stack = 8;        // LDS stack entries per thread (power of two)
-	int fetch_min = 16;       // idle lanes that trigger a batch fetch (64 = no compaction)
-	int steps = 8;            // traversal steps between fetches
+	int chunk = 64;           // rays a wave reserves per atomic (refilled lane by lane)
+	int leaf_phase = 32;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int count_stats = 0;      // count node visits and triangle tests
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)
@@ -95,6 +95,9 @@ struct TraceScratch {
 	int n_cus = 0;
 	uint32_t *host_err = nullptr;  // pinned copy of the last launch's error flags
 };
+
+// Reads the error flags of the last traversal launched with `s` (synchronises `st`).
+hipError_t trace_errors(TraceScratch &s, hipStream_t st, uint32_t *flags);
 
 // Traces n device rays against `b`, writing HitReg records at `out` (device) with the layout
 // of `mask`. Launches on `st`; does not synchronise. `ms` receives the duration of the

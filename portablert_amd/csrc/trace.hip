@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "common.h"
 #include "internal.h"
@@ -37,11 +38,12 @@ struct KArgs {
 	uint32_t stride;
 	uint32_t mask;
 	uint32_t spill_depth;
-	uint32_t fetch_min;
-	uint32_t steps;
+	uint32_t chunk;         // rays a wave reserves per atomic
+	uint32_t leaf_phase;    // triangle phase once parked lanes >= live lanes * leaf_phase / 64
 	float margin_rel;
 	float margin_abs;
 	int cull;
+	int fast_scene;         // every vertex coordinate below 2^60 in magnitude
 };
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
@@ -54,9 +56,11 @@ struct Ray {
 	float abs_margin;  // margin_abs / |d|
 	bool degen;        // some 1/d is infinite: axis-parallel handling in node tests
 	bool patho;        // non-finite origin or NaN direction: every node passes
+	bool fast;         // |1/d| and |o| below 2^60 in a scene within 2^60: one-fma node bounds
 };
 
-__device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float margin_abs, Ray &r) {
+__device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float margin_abs, bool fast_scene,
+                                         Ray &r) {
 	const float *p = rays + 6 * idx;  // 4-byte aligned AoS records of 24 bytes
 #pragma unroll
 	for (int k = 0; k < 3; ++k) {
@@ -68,6 +72,9 @@ __device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float 
 	r.degen = !(__builtin_isfinite(r.inv[0]) && __builtin_isfinite(r.inv[1]) && __builtin_isfinite(r.inv[2]));
 	r.patho = !(__builtin_isfinite(r.o[0]) && __builtin_isfinite(r.o[1]) && __builtin_isfinite(r.o[2])) ||
 	          __builtin_isnan(r.d[0]) || __builtin_isnan(r.d[1]) || __builtin_isnan(r.d[2]);
+	const float big = 0x1p60f;
+	r.fast = fast_scene && !r.degen && !r.patho && fabsf(r.inv[0]) < big && fabsf(r.inv[1]) < big &&
+	         fabsf(r.inv[2]) < big && fabsf(r.o[0]) < big && fabsf(r.o[1]) < big && fabsf(r.o[2]) < big;
 	const float len = sqrtf(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
 	r.abs_margin = len > 0.0f ? margin_abs / len : __builtin_inff();
 	r.tb = __builtin_inff();
@@ -137,35 +144,121 @@ struct Stack {
 	}
 };
 
+// Sort key of a child hit: the order-preserving integer image of its entry distance with the
+// low three bits replaced by the child slot. Clearing bits only lowers the distance the key
+// decodes to, so re-culling with the decoded value stays conservative. -inf (a box that must
+// pass) is clamped to -FLT_MAX, which every culling limit admits. Misses sort last.
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t sort_key(float tmin, int slot) {
+	const uint32_t u = __float_as_uint(fmaxf(tmin, -3.402823466e38f));
+	const uint32_t o = u ^ (static_cast<uint32_t>(static_cast<int32_t>(u) >> 31) | 0x80000000u);
+	return (o & ~7u) | static_cast<uint32_t>(slot);
+}
+
+__device__ __forceinline__ float key_tmin(uint32_t key) {
+	const uint32_t o = key & ~7u;
+	return __uint_as_float(o ^ (static_cast<uint32_t>(~static_cast<int32_t>(o) >> 31) | 0x80000000u));
+}
+
 #define PRT_CE(i, j)                                                                               \
-	if (key[j] < key[i]) {                                                                         \
-		const float tk = key[i];                                                                   \
-		key[i] = key[j];                                                                           \
-		key[j] = tk;                                                                               \
-		const uint32_t tv = val[i];                                                                \
-		val[i] = val[j];                                                                           \
-		val[j] = tv;                                                                               \
+	{                                                                                              \
+		const uint32_t lo_ = min(k[i], k[j]);                                                      \
+		k[j] = max(k[i], k[j]);                                                                    \
+		k[i] = lo_;                                                                                \
 	}
 
-// Tests the children of one node; returns them sorted by entry distance in key/val
-// (val = kInvalid for misses, which sort last with key = +inf).
+// Sorting network for 8 keys (19 compare-exchanges).
+__device__ __forceinline__ void sort8(uint32_t k[8]) {
+	PRT_CE(0, 2) PRT_CE(1, 3) PRT_CE(4, 6) PRT_CE(5, 7) PRT_CE(0, 4) PRT_CE(1, 5) PRT_CE(2, 6) PRT_CE(3, 7)
+	PRT_CE(0, 1) PRT_CE(2, 3) PRT_CE(4, 5) PRT_CE(6, 7) PRT_CE(2, 4) PRT_CE(3, 5) PRT_CE(1, 4) PRT_CE(3, 6)
+	PRT_CE(1, 2) PRT_CE(3, 4) PRT_CE(5, 6)
+}
+#undef PRT_CE
+
+// Children of one node that pass the box test, sorted by entry distance, plus what is needed
+// to turn a key back into a stack entry.
+struct NodeHits {
+	uint32_t key[8];
+	uint32_t child_base, tri_base, meta_lo, meta_hi;
+	__device__ __forceinline__ uint32_t entry(uint32_t key) const {
+		const uint32_t slot = key & 7u;
+		const uint32_t meta = ((slot < 4u ? meta_lo : meta_hi) >> (8u * (slot & 3u))) & 0xFFu;
+		return meta_internal(meta) ? child_base + meta_offset(meta)
+		                           : leaf_entry(tri_base + meta_offset(meta), meta_count(meta));
+	}
+};
+
+// Tests the children of one node (fast_ok selects the one-fma bound evaluation).
 template <class Node>
 __device__ __forceinline__ void test_children(const Node *nodes, uint32_t idx, const Ray &r, float limit,
-                                              float key[8], uint32_t val[8]);
+                                              bool fast_ok, NodeHits &h);
+
+// Byte k of a little-endian dword pair, as a float (one v_cvt_f32_ubyteN when k is constant).
+__device__ __forceinline__ float byte_f(const uint32_t w[2], int k) {
+	return static_cast<float>((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+}
 
 template <>
 __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_t idx, const Ray &r, float limit,
-                                                     float key[8], uint32_t val[8]) {
+                                                     bool fast_ok, NodeHits &h) {
 	const uint4 *src = reinterpret_cast<const uint4 *>(nodes + idx);
 	const uint4 w0 = src[0], w1 = src[1], w2 = src[2], w3 = src[3], w4 = src[4];
 	const float p[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
 	const float s[3] = {exp2i(static_cast<int8_t>(w0.w & 0xFF)), exp2i(static_cast<int8_t>((w0.w >> 8) & 0xFF)),
 	                    exp2i(static_cast<int8_t>((w0.w >> 16) & 0xFF))};
-	const uint32_t child_base = w1.x, tri_base = w1.y;
+	h.child_base = w1.x;
+	h.tri_base = w1.y;
 	const uint32_t meta_lo = w1.z, meta_hi = w1.w;
+	h.meta_lo = meta_lo;
+	h.meta_hi = meta_hi;
 	// qlo x: w2.x w2.y, qlo y: w2.z w2.w, qlo z: w3.x w3.y, qhi x: w3.z w3.w, qhi y: w4.x w4.y, qhi z: w4.z w4.w
 	const uint32_t ql[3][2] = {{w2.x, w2.y}, {w2.z, w2.w}, {w3.x, w3.y}};
 	const uint32_t qh[3][2] = {{w3.z, w3.w}, {w4.x, w4.y}, {w4.z, w4.w}};
+
+	// Fast test: on axis a the child bound q lies at t = q * step_a + base_a in real arithmetic,
+	// with step_a = 2^e_a / d_a and base_a = (p_a - o_a) / d_a. One fma per bound replaces the
+	// dequantize / subtract / multiply of the reference formula. Both fp32 evaluations stay
+	// within 8 * 2^-24 * |1/d_a| * (255 * 2^e_a + |p_a| + |o_a|) of the real value, so moving the
+	// near plane in and the far plane out by 2^-18 times that amount keeps every fast interval
+	// around the interval the reference formula gives for the same dequantized box (which the
+	// builder already rounded outward). fast_ok guarantees |1/d|, |o| and the scene coordinates
+	// are below 2^60, so none of these values overflows; other rays take the reference formula.
+	if (fast_ok) {
+		float step[3], bnear[3], bfar[3];
+		uint32_t qn[3][2], qf[3][2];
+#pragma unroll
+		for (int ax = 0; ax < 3; ++ax) {
+			step[ax] = s[ax] * r.inv[ax];
+			const float base = (p[ax] - r.o[ax]) * r.inv[ax];
+			const float margin = (0x1p-18f * fabsf(r.inv[ax])) * fmaf(255.0f, s[ax], fabsf(p[ax]) + fabsf(r.o[ax]));
+			bnear[ax] = base - margin;
+			bfar[ax] = base + margin;
+			// the near plane is the low bound for 1/d >= 0, the high bound otherwise
+			const bool pos = r.inv[ax] >= 0.0f;
+			qn[ax][0] = pos ? ql[ax][0] : qh[ax][0];
+			qn[ax][1] = pos ? ql[ax][1] : qh[ax][1];
+			qf[ax][0] = pos ? qh[ax][0] : ql[ax][0];
+			qf[ax][1] = pos ? qh[ax][1] : ql[ax][1];
+		}
+#pragma unroll
+		for (int k = 0; k < 8; ++k) {
+			const uint32_t meta = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+			float tn[3], tf[3];
+#pragma unroll
+			for (int ax = 0; ax < 3; ++ax) {
+				tn[ax] = fmaf(byte_f(qn[ax], k), step[ax], bnear[ax]);
+				tf[ax] = fmaf(byte_f(qf[ax], k), step[ax], bfar[ax]);
+			}
+			const float tmin = fmaxf(fmaxf(tn[0], tn[1]), tn[2]);
+			const float tmax = fminf(fminf(tf[0], tf[1]), tf[2]);
+			const bool hit = meta != 0u && tmax >= 0.0f && tmin <= fminf(tmax, limit);
+			h.key[k] = hit ? sort_key(tmin, k) : kMiss;
+		}
+		sort8(h.key);
+		return;
+	}
+	// Reference formula on the dequantized box (also the path of the exact mode).
 #pragma unroll
 	for (int k = 0; k < 8; ++k) {
 		const uint32_t meta = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
@@ -179,22 +272,20 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 		}
 		const float tmin = child_slab(r, lo, hi, limit);
 		const bool hit = meta != 0u && !__builtin_isnan(tmin);
-		key[k] = hit ? tmin : __builtin_inff();
-		val[k] = !hit ? kInvalid
-		              : (meta_internal(meta) ? child_base + meta_offset(meta)
-		                                     : leaf_entry(tri_base + meta_offset(meta), meta_count(meta)));
+		h.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
-	PRT_CE(0, 2) PRT_CE(1, 3) PRT_CE(4, 6) PRT_CE(5, 7) PRT_CE(0, 4) PRT_CE(1, 5) PRT_CE(2, 6) PRT_CE(3, 7)
-	PRT_CE(0, 1) PRT_CE(2, 3) PRT_CE(4, 5) PRT_CE(6, 7) PRT_CE(2, 4) PRT_CE(3, 5) PRT_CE(1, 4) PRT_CE(3, 6)
-	PRT_CE(1, 2) PRT_CE(3, 4) PRT_CE(5, 6)
+	sort8(h.key);
 }
 
 template <>
 __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_t idx, const Ray &r, float limit,
-                                                     float key[8], uint32_t val[8]) {
+                                                     bool /*fast_ok*/, NodeHits &hh) {
 	const NodeF *n = nodes + idx;
 	const uint4 h = *reinterpret_cast<const uint4 *>(n);
-	const uint32_t child_base = h.x, tri_base = h.y;
+	hh.child_base = h.x;
+	hh.tri_base = h.y;
+	hh.meta_lo = h.z;
+	hh.meta_hi = h.w;
 	float lo[3][8], hi[3][8];
 #pragma unroll
 	for (int ax = 0; ax < 3; ++ax) {
@@ -213,16 +304,10 @@ __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_
 		const float u[3] = {hi[0][k], hi[1][k], hi[2][k]};
 		const float tmin = child_slab(r, l, u, limit);
 		const bool hit = meta != 0u && !__builtin_isnan(tmin);
-		key[k] = hit ? tmin : __builtin_inff();
-		val[k] = !hit ? kInvalid
-		              : (meta_internal(meta) ? child_base + meta_offset(meta)
-		                                     : leaf_entry(tri_base + meta_offset(meta), meta_count(meta)));
+		hh.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
-	PRT_CE(0, 2) PRT_CE(1, 3) PRT_CE(4, 6) PRT_CE(5, 7) PRT_CE(0, 4) PRT_CE(1, 5) PRT_CE(2, 6) PRT_CE(3, 7)
-	PRT_CE(0, 1) PRT_CE(2, 3) PRT_CE(4, 5) PRT_CE(6, 7) PRT_CE(2, 4) PRT_CE(3, 5) PRT_CE(1, 4) PRT_CE(3, 6)
-	PRT_CE(1, 2) PRT_CE(3, 4) PRT_CE(5, 6)
+	sort8(hh.key);
 }
-#undef PRT_CE
 
 __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ray &r) {
 	const uint32_t first = entry & kTriMask;
@@ -257,90 +342,106 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 
 	Ray r;
 	uint64_t ray_idx = ~0ull;
-	bool active = false;   // lane holds a ray
-	bool have_cur = false; // cur is the next entry to process
-	uint32_t cur = 0;
-	bool exhausted = false;  // wave-uniform: no rays left to fetch
+	bool active = false;       // lane holds a ray
+	uint32_t cur = kInvalid;   // next entry to process (node or leaf)
+	uint32_t pend = kInvalid;  // parked leaf, tested in the next triangle phase
+	// wave-uniform ray chunk [chunk_next, chunk_end) reserved with one atomic
+	uint64_t chunk_next = 0, chunk_end = 0;
+	bool exhausted = false;
 	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0;
 
 	while (true) {
-		// ---- refill finished lanes (wave-uniform control flow)
-		if (!exhausted) {
-			const uint64_t need = __ballot(!active);
-			const uint32_t n_need = __popcll(need);
-			const uint32_t n_act = 64u - n_need;
-			if (n_need >= a.fetch_min || n_act == 0u) {
-				const int leader = __ffsll(static_cast<unsigned long long>(need)) - 1;
+		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle)
+		const uint64_t need = __ballot(!active);
+		if (need != 0ull && !exhausted) {
+			if (chunk_next >= chunk_end) {
 				uint64_t base = 0;
-				if (static_cast<int>(lane) == leader)
-					base = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter), static_cast<unsigned long long>(n_need));
-				base = __shfl(base, leader);
-				if (!active) {
-					const uint64_t idx = base + __popcll(need & lt_mask);
-					if (idx < a.n_rays) {
-						ray_idx = idx;
-						load_ray(a.rays, idx, a.margin_abs, r);
-						active = true;
-						have_cur = a.n_nodes > 0;
-						cur = 0;
-						st.sp = 0;
-					}
-				}
-				if (base + n_need >= a.n_rays) exhausted = true;
+				if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter),
+				                                static_cast<unsigned long long>(a.chunk));
+				base = __shfl(base, 0);
+				chunk_next = base;
+				chunk_end = base + a.chunk < a.n_rays ? base + a.chunk : a.n_rays;
+				if (base >= a.n_rays) exhausted = true;
+			}
+			const uint64_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0ull;
+			const uint32_t rank = __popcll(need & lt_mask);
+			if (!active && rank < avail) {
+				ray_idx = chunk_next + rank;
+				load_ray(a.rays, ray_idx, a.margin_abs, a.fast_scene != 0, r);
+				active = true;
+				cur = a.n_nodes > 0 ? 0u : kInvalid;
+				pend = kInvalid;
+				st.sp = 0;
+			}
+			const uint64_t took = __popcll(need) < avail ? __popcll(need) : avail;
+			chunk_next += took;
+		}
+		if (__ballot(active) == 0ull) {
+			if (exhausted || chunk_next >= chunk_end) {
+				if (exhausted) break;
+				continue;  // chunk used up with every lane idle: reserve the next one
 			}
 		}
-		if (__ballot(active) == 0ull) break;
+		if (!active) continue;
 
-		for (uint32_t step = 0; step < a.steps && active; ++step) {
-			if (!have_cur) {
-				const float limit = cull_limit(r, a);
-				while (st.sp > 0) {
-					const uint2 e = st.pop();
-					if (__uint_as_float(e.y) <= limit) {
-						cur = e.x;
-						have_cur = true;
-						break;
-					}
-				}
-			}
-			if (!have_cur) {
-				// ray finished
-				Hit h;
-				h.t = r.tb;
-				h.u = r.ub;
-				h.v = r.vb;
-				h.id = r.idb;
-				write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
-				active = false;
-				break;
-			}
-			if (cur & kLeafBit) {
-				if ((cur & kTriMask) + ((cur >> 29) & 3u) + 1u > a.n_tris) {  // corrupt entry: report
+		// ---- fetch work: pop one entry; park a leaf if the parking slot is free
+		if (cur == kInvalid && st.sp > 0) {
+			const uint2 e = st.pop();
+			if (__uint_as_float(e.y) <= cull_limit(r, a)) cur = e.x;
+		}
+		if (cur != kInvalid && (cur & kLeafBit) && pend == kInvalid) {
+			pend = cur;
+			cur = kInvalid;
+		}
+		if (cur == kInvalid && pend == kInvalid && st.sp == 0) {  // ray finished
+			Hit h;
+			h.t = r.tb;
+			h.u = r.ub;
+			h.v = r.vb;
+			h.id = r.idb;
+			write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
+			active = false;
+			continue;
+		}
+
+		// ---- one phase per iteration for the whole wave: nodes, or one triangle per parked
+		// leaf. Lanes with a parked leaf keep traversing nodes meanwhile; the triangle phase
+		// runs when no lane has a node to visit or enough lanes have parked a leaf.
+		const uint64_t want_node = __ballot(cur != kInvalid && !(cur & kLeafBit));
+		const uint64_t parked = __ballot(pend != kInvalid);
+		const uint64_t live = __ballot(true);
+		const bool tri_phase = want_node == 0ull || __popcll(parked) * 64u >= __popcll(live) * a.leaf_phase;
+		if (tri_phase) {
+			if (pend != kInvalid) {
+				const uint32_t tri = pend & kTriMask;
+				const uint32_t left = ((pend >> 29) & 3u) + 1u;
+				if (tri + left > a.n_tris) {  // corrupt entry: report, never read
 					atomicOr(a.counter + 8, 1u);
-					have_cur = false;
+					pend = kInvalid;
 					continue;
 				}
-				test_leaf(a.tris, cur, r);
-				if (COUNT) {
-					++c_leaves;
-					c_tris += ((cur >> 29) & 3u) + 1u;
-				}
-				have_cur = false;
-			} else {
-				if (cur >= a.n_nodes || st.sp + 7u > a.max_entries) {  // corrupt node or stack bound
-					atomicOr(a.counter + 8, 2u);
-					have_cur = false;
-					continue;
-				}
-				float key[8];
-				uint32_t val[8];
-				test_children<Node>(nodes, cur, r, cull_limit(r, a), key, val);
-				if (COUNT) ++c_nodes;
-				have_cur = val[0] != kInvalid;
-				cur = val[0];
+				test_leaf(a.tris, leaf_entry(tri, 1u), r);
+				if (COUNT) ++c_tris;
+				pend = left == 1u ? kInvalid : pend + 1u - (1u << 29);  // next triangle, one fewer left
+			}
+		} else if (cur != kInvalid && !(cur & kLeafBit)) {
+			if (cur >= a.n_nodes || st.sp + 7u > a.max_entries) {  // corrupt node or stack bound
+				atomicOr(a.counter + 8, 2u);
+				cur = kInvalid;
+				continue;
+			}
+			NodeHits h;
+			test_children<Node>(nodes, cur, r, cull_limit(r, a), a.cull && r.fast, h);
+			if (COUNT) ++c_nodes;
+			cur = h.key[0] != kMiss ? h.entry(h.key[0]) : kInvalid;
+			// farthest first, so the nearest remaining child is on top
 #pragma unroll
-				for (int k = 7; k >= 1; --k)
-					if (val[k] != kInvalid) st.push(val[k], key[k]);
+			for (int k = 7; k >= 1; --k) {
+				if (h.key[k] != kMiss) {
+					const uint32_t e = h.entry(h.key[k]);
+					if (COUNT) c_leaves += (e & kLeafBit) ? 1u : 0u;
+					st.push(e, key_tmin(h.key[k]));
+				}
 			}
 		}
 	}
@@ -425,11 +526,13 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.stride = static_cast<uint32_t>(layout_of(mask).size);
 	a.mask = mask;
 	a.spill_depth = spill_depth;
-	a.fetch_min = static_cast<uint32_t>(std::max(1, std::min(64, p.fetch_min)));
-	a.steps = static_cast<uint32_t>(std::max(1, p.steps));
+	a.chunk = static_cast<uint32_t>(std::max(64, std::min(1 << 16, p.chunk)));
+	a.leaf_phase = static_cast<uint32_t>(std::max(1, std::min(64, p.leaf_phase)));
 	a.margin_rel = p.margin_rel;
 	a.margin_abs = p.margin_abs;
 	a.cull = p.cull;
+	a.fast_scene = 1;
+	for (int k = 0; k < 6; ++k) a.fast_scene &= std::fabs(b.scene_box[k]) < 0x1p60f ? 1 : 0;
 
 	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 5);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -442,11 +545,6 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	hipError_t e = p.node_format == 0 ? launch_b<NodeF>(a, grid, B, S, count, st)
 	                                  : launch_b<NodeQ>(a, grid, B, S, count, st);
 	PRT_TRY(e);
-	if (!s.host_err) {
-		PRT_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.host_err), sizeof(uint32_t)));
-		*s.host_err = 0;
-	}
-	PRT_TRY(hipMemcpyAsync(s.host_err, a.counter + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 	if (ms) {
 		PRT_TRY(hipEventRecord(e1, st));
 		PRT_TRY(hipEventSynchronize(e1));
@@ -465,6 +563,19 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	}
 	return hipSuccess;
 #undef PRT_TRY
+}
+
+hipError_t trace_errors(TraceScratch &s, hipStream_t st, uint32_t *flags) {
+	*flags = 0;
+	if (!s.counter.ptr) return hipSuccess;
+	if (!s.host_err) {
+		hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&s.host_err), sizeof(uint32_t));
+		if (e != hipSuccess) return e;
+	}
+	hipError_t e = hipMemcpyAsync(s.host_err, s.counter.as<uint32_t>() + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+	if (e == hipSuccess) e = hipStreamSynchronize(st);
+	if (e == hipSuccess) *flags = *s.host_err;
+	return e;
 }
 
 }  // namespace prt
