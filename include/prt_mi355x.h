@@ -62,7 +62,10 @@ enum {
 	PRT_STAT_LEAF_VISITS = 7,
 	PRT_STAT_NODE_BYTES = 8,  /* bytes per node of the selected format */
 	PRT_STAT_API_MS = 9,      /* last host-buffer nearest_hits, wall time incl. copies */
-	PRT_STAT_N_TRIS = 10
+	PRT_STAT_N_TRIS = 10,
+	PRT_STAT_NODE_PHASES = 11, /* wave iterations spent in node phases (with COUNT_STATS) */
+	PRT_STAT_TRI_PHASES = 12,  /* wave iterations spent in triangle phases */
+	PRT_STAT_CULLED_POPS = 13  /* stack entries discarded by distance culling */
 };
 
 /* Number of usable gfx950 devices and the name of device 0 (name may be NULL). Returns PRT_OK

@@ -351,6 +351,9 @@ int prt_mi355x_get_stat(const prt_ctx *c, int stat, double *v) {
 	case PRT_STAT_NODE_BYTES: *v = c->tp.node_format == 0 ? sizeof(prt::NodeF) : sizeof(prt::NodeQ); break;
 	case PRT_STAT_API_MS: *v = c->api_ms; break;
 	case PRT_STAT_N_TRIS: *v = static_cast<double>(c->bvh.n_tris); break;
+	case PRT_STAT_NODE_PHASES: *v = static_cast<double>(c->stats.node_phases); break;
+	case PRT_STAT_TRI_PHASES: *v = static_cast<double>(c->stats.tri_phases); break;
+	case PRT_STAT_CULLED_POPS: *v = static_cast<double>(c->stats.culled_pops); break;
 	default: return PRT_ERR_ARG;
 	}
 	return PRT_OK;

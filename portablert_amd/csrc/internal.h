@@ -74,11 +74,11 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n, const BuildParams &p, Bvh8 &o
 struct TraceParams {
 	int node_format = 1;      // 0 fp32 nodes, 1 8-bit quantized nodes
 	int cull = 1;             // 0 exact (no distance culling), 1 distance culling with margin
-	int block = 256;          // threads per workgroup
-	int waves_per_cu = 32;    // persistent waves per CU
-	int lds_stack = 8;        // LDS stack entries per thread (power of two)
+	int block = 128;          // threads per workgroup
+	int waves_per_cu = 24;    // persistent waves per CU
+	int lds_stack = 16;       // LDS stack entries per thread (power of two)
 	int chunk = 64;           // rays a wave reserves per atomic (refilled lane by lane)
-	int leaf_phase = 32;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
+	int leaf_phase = 16;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int count_stats = 0;      // count node visits and triangle tests
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)
@@ -87,6 +87,7 @@ struct TraceParams {
 struct TraceStats {
 	unsigned long long node_visits = 0, tri_tests = 0, leaf_visits = 0;
 	uint32_t errors = 0;  // bit 0 corrupt leaf entry, bit 1 corrupt node / stack bound
+	unsigned long long node_phases = 0, tri_phases = 0, culled_pops = 0;  // wave iterations
 };
 
 struct TraceScratch {

@@ -348,7 +348,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 	// wave-uniform ray chunk [chunk_next, chunk_end) reserved with one atomic
 	uint64_t chunk_next = 0, chunk_end = 0;
 	bool exhausted = false;
-	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0;
+	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0, c_nphase = 0, c_tphase = 0, c_culled = 0;
 
 	while (true) {
 		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle)
@@ -386,8 +386,12 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 
 		// ---- fetch work: pop one entry; park a leaf if the parking slot is free
 		if (cur == kInvalid && st.sp > 0) {
-			const uint2 e = st.pop();
-			if (__uint_as_float(e.y) <= cull_limit(r, a)) cur = e.x;
+			const float limit = cull_limit(r, a);
+			do {
+				const uint2 e = st.pop();
+				if (__uint_as_float(e.y) <= limit) cur = e.x;
+				else if (COUNT) ++c_culled;
+			} while (cur == kInvalid && st.sp > 0);
 		}
 		if (cur != kInvalid && (cur & kLeafBit) && pend == kInvalid) {
 			pend = cur;
@@ -411,6 +415,10 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 		const uint64_t parked = __ballot(pend != kInvalid);
 		const uint64_t live = __ballot(true);
 		const bool tri_phase = want_node == 0ull || __popcll(parked) * 64u >= __popcll(live) * a.leaf_phase;
+		if (COUNT && lane == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(live)) - 1)) {
+			if (tri_phase) ++c_tphase;
+			else ++c_nphase;
+		}
 		if (tri_phase) {
 			if (pend != kInvalid) {
 				const uint32_t tri = pend & kTriMask;
@@ -449,6 +457,9 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 1, c_nodes);
 		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 2, c_tris);
 		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 3, c_leaves);
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 5, c_nphase);
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 6, c_tphase);
+		atomicAdd(reinterpret_cast<unsigned long long *>(a.counter) + 7, c_culled);
 	}
 }
 
@@ -509,7 +520,7 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	// stack bound: each visited node pushes at most 7 entries, so 7 * depth + 1 suffice
 	const uint32_t max_entries = 7u * std::max(1u, b.depth) + 8u;
 	const uint32_t spill_depth = max_entries > static_cast<uint32_t>(S) ? max_entries - S : 1u;
-	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 5));
+	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 8));
 	PRT_TRY(s.spill.ensure(sizeof(uint2) * static_cast<uint64_t>(spill_depth) * grid * B));
 
 	KArgs a;
@@ -534,7 +545,7 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.fast_scene = 1;
 	for (int k = 0; k < 6; ++k) a.fast_scene &= std::fabs(b.scene_box[k]) < 0x1p60f ? 1 : 0;
 
-	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 5);
+	hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long *>(a.counter), 8);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (ms) {
 		PRT_TRY(hipEventCreate(&e0));
@@ -553,13 +564,16 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 		(void)hipEventDestroy(e1);
 	}
 	if (stats) {
-		unsigned long long h[5];
+		unsigned long long h[8];
 		PRT_TRY(hipMemcpyAsync(h, a.counter, sizeof(h), hipMemcpyDeviceToHost, st));
 		PRT_TRY(hipStreamSynchronize(st));
 		stats->node_visits = h[1];
 		stats->tri_tests = h[2];
 		stats->leaf_visits = h[3];
 		stats->errors = static_cast<uint32_t>(h[4]);
+		stats->node_phases = h[5];
+		stats->tri_phases = h[6];
+		stats->culled_pops = h[7];
 	}
 	return hipSuccess;
 #undef PRT_TRY

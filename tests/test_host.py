@@ -1,0 +1,143 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every entry point
+include/prt_mi355x.h declares, the HitReg layouts match the reference headers for all 32
+field masks, the no-device behaviour of the API, the deterministic workload generators, and
+the C++ API headers compiling (and, without a device, taking the no-backend path)."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import portablert_amd as prt
+from portablert_amd import scenes
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "prt_mi355x.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(prt_mi355x_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert len(syms) >= 14
+    lib = ctypes.CDLL(prt.LIB)
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_layouts_match_reference_headers():
+    ref = np.load(os.path.join(GOLDEN, "hitreg_layouts.npy"))
+    for mask in range(32):
+        size, off = prt.layout(mask)
+        assert [size] + off == list(ref[mask]), (mask, size, off, ref[mask])
+
+
+def test_layout_sizes_of_survey():
+    assert prt.layout(31)[0] == 32
+    assert prt.layout(prt.T | prt.PRIMITIVE_ID) == (16, [-1, -1, 4, 8, -1, -1, -1, -1])
+    assert prt.layout(prt.T)[0] == 16
+    assert prt.layout(prt.VALID)[0] == 8
+
+
+def _gpu_present():
+    return prt.probe()[0] > 0
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-device behaviour")
+def test_no_device_behaviour():
+    n, name = prt.probe()
+    assert n == 0 and name == ""
+    with pytest.raises(prt.PrtError) as e:
+        prt.Context(0)
+    assert e.value.code == prt.ERR_NODEV
+    b = prt.MI355XBackend()
+    assert not b.is_available()
+
+
+def test_null_and_bad_arguments_need_no_device():
+    lib = prt.lib()
+    assert lib.prt_mi355x_probe(None, None, 0) == prt.ERR_ARG
+    out = np.zeros(9, np.int32)
+    assert lib.prt_mi355x_layout(32, out.ctypes.data_as(ctypes.c_void_p)) == prt.ERR_ARG
+    assert lib.prt_mi355x_set_tris(None, None, 0) == prt.ERR_ARG
+    assert lib.prt_mi355x_nearest_hits(None, None, 0, 0, None, 0, None) == prt.ERR_ARG
+    assert lib.prt_mi355x_last_error(None) == b"null context"
+
+
+def test_generators_are_deterministic_and_sized():
+    for w, n in ((0, 16384), (1, 81920), (2, 262144)):
+        a, b = scenes.scene(w), scenes.scene(w)
+        assert a.shape == (n, 9) and np.array_equal(a, b)
+        assert np.isfinite(a).all()
+    assert scenes.n_rays(2) == 8388608 and scenes.n_rays(0) == 1 << 20
+    r = scenes.rays(2, first=1000, count=50)
+    assert np.array_equal(r, scenes.ray_sample(2, np.arange(1000, 1050)))
+    assert scenes.n_scenes(3) == 64
+    counts = [scenes.lib().prt_scene_tris(3, i) for i in range(64)]
+    assert min(counts) >= 1000 and max(counts) <= 200000
+    rc = [scenes.n_rays(3, i) for i in range(64)]
+    assert min(rc) >= 10000 and max(rc) <= 2000000
+
+
+def test_room_scene_shape():
+    t = scenes.scene(2).reshape(-1, 3, 3)
+    assert (t.min(axis=(0, 1)) >= -1e-6).all() and (t.max(axis=(0, 1)) <= np.array([30, 12, 18]) + 1e-5).all()
+    a = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)
+    med = float(np.median(np.log(a[a > 0])))
+    assert -9 < med < -4  # log-normal areas around mu = -6 (caps and clamps move it a little)
+
+
+def test_pinhole_directions_unit_and_centered():
+    r = scenes.rays(0)
+    d = r[:, 3:]
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert (r[:, :3] == np.array([0, 0, -3], np.float32)).all()
+    mid = d[512 * 1024 + 512]
+    assert abs(mid[0]) < 2e-3 and abs(mid[1]) < 2e-3 and mid[2] > 0.99
+
+
+def test_bounce_rays_leave_the_surface():
+    import oracle
+    tris = scenes.scene(1)
+    prim = scenes.rays(1, first=1920 * 540, count=1920)
+    hits = oracle.trace(tris, prim)
+    p = np.stack([hits["px"], hits["py"], hits["pz"]], 1)
+    b = scenes.bounce_rays(tris, prim, hits["t"], hits["primitive_id"], p)
+    nhit = int((hits["t"] < np.inf).sum())
+    assert len(b) == 4 * nhit and nhit > 0
+    assert np.allclose(np.linalg.norm(b[:, 3:], axis=1), 1.0, atol=1e-5)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no C++ compiler")
+def test_cpp_api_compiles_and_runs(tmp_path):
+    exe = tmp_path / "api_test"
+    libdir = os.path.dirname(prt.LIB)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "api_test.cpp"), "-L", libdir, "-lprt_mi355x",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    if not _gpu_present():
+        r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "no-backend path" in r.stdout
+
+
+REF_EXAMPLES = "/root/reference/examples"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference sources not present")
+@pytest.mark.parametrize("name", ["triangle", "simple", "backends"])
+def test_reference_examples_compile_unchanged(name, tmp_path):
+    """The reference's own example programs build against these headers without edits."""
+    libdir = os.path.dirname(prt.LIB)
+    exe = tmp_path / name
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(REF_EXAMPLES, name, "main.cpp"), "-L", libdir, "-lprt_mi355x",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
