@@ -65,7 +65,14 @@ enum {
 	PRT_STAT_N_TRIS = 10,
 	PRT_STAT_NODE_PHASES = 11, /* wave iterations spent in node phases (with COUNT_STATS) */
 	PRT_STAT_TRI_PHASES = 12,  /* wave iterations spent in triangle phases */
-	PRT_STAT_CULLED_POPS = 13  /* stack entries discarded by distance culling */
+	PRT_STAT_CULLED_POPS = 13, /* stack entries discarded by distance culling */
+	PRT_STAT_MLP_MS = 14,      /* last learned call: encoding + MLP + top-4 (PRT_OPT_TIME_KERNEL) */
+	PRT_STAT_SEEDED_MS = 15,   /* last learned call: seeded traversal kernel */
+	PRT_STAT_TRAIN_MS = 16,    /* last learned_train, wall time */
+	PRT_STAT_TRAIN_LOSS = 17,  /* cross-entropy of the last training batch */
+	PRT_STAT_CLUSTERS = 18,    /* BVH8 subtree clusters */
+	PRT_STAT_SEED_TOP1 = 19,   /* last learned_eval: fraction of rays whose top-1 cluster is true */
+	PRT_STAT_SEED_TOP4 = 20    /* ... whose true cluster is among the top 4 */
 };
 
 /* Number of usable gfx950 devices and the name of device 0 (name may be NULL). Returns PRT_OK
@@ -120,6 +127,38 @@ int prt_mi355x_layout(uint32_t mask, int32_t out[9]);
  * nodes, 2 leaf triangle records. *bytes is the size; with out == NULL only the size is
  * returned. */
 int prt_mi355x_export(prt_ctx *ctx, int which, void *out, uint64_t *bytes);
+
+/* ---- Learned t_max seeding. No counterpart in the reference: a BF16 MLP (192 -> 256 -> 256 ->
+ * 256 -> 4097, MFMA kernels) predicts the BVH8 subtree cluster holding each ray's hit; the 4
+ * best clusters are intersected exactly before the full traversal, so results are identical to
+ * prt_mi355x_nearest_hits_device. */
+
+/* Cuts the current BVH into at most 4096 subtree clusters and initialises the network
+ * (deterministic from `seed`). Call after every set_tris. */
+int prt_mi355x_learned_setup(prt_ctx *ctx, uint64_t seed);
+
+/* Traces n device rays with the plain traversal, labels each with its hit cluster (or miss)
+ * and trains the network for `epochs` passes in batches of `batch` rays (multiple of 32), Adam
+ * with learning rate lr. */
+int prt_mi355x_learned_train(prt_ctx *ctx, const float *d_rays6, uint64_t n, int epochs, int batch, float lr);
+
+/* prt_mi355x_nearest_hits_device with learned seeding (same records, same arguments). */
+int prt_mi355x_nearest_hits_learned_device(prt_ctx *ctx, const float *d_rays6, uint64_t n, uint32_t mask,
+                                           void *d_out, uint32_t stride, const int32_t off[8], void *stream);
+
+/* Fraction of n device rays whose true cluster is the top-1 / among the top-4 predictions. */
+int prt_mi355x_learned_eval(prt_ctx *ctx, const float *d_rays6, uint64_t n, double *top1, double *top4);
+
+/* Number of clusters and (tri_cluster != NULL) the cluster of every primitive id. */
+int prt_mi355x_learned_clusters(prt_ctx *ctx, uint32_t *count, uint32_t *tri_cluster);
+
+/* Network access for checking the kernels on the host: parameters of layer 0..3 (W^T [out][in]
+ * and bias, fp32; out = 4097 for the last layer), gradients of one batch, inference. */
+int prt_mi355x_mlp_params(prt_ctx *ctx, int layer, int get, float *w, float *b);
+int prt_mi355x_mlp_grad(prt_ctx *ctx, const float *rays6, uint64_t n, const uint32_t *labels, float *loss,
+                        uint16_t *x0_bf16);
+int prt_mi355x_mlp_grads(prt_ctx *ctx, int layer, float *gw, float *gb);
+int prt_mi355x_mlp_top4(prt_ctx *ctx, const float *rays6, uint64_t n, uint32_t *top4, float *logits);
 
 #ifdef __cplusplus
 }

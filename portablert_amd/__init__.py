@@ -281,3 +281,101 @@ def nearest_hits(rays, *tags):
     if b is None:
         raise RuntimeError("Unknown backend")
     return b.nearest_hits(rays, *tags)
+
+
+# ------------------------------------------------------------------ learned t_max seeding
+
+MLP_IN, MLP_HIDDEN, CLUSTERS = 192, 256, 4096
+MLP_OUT = CLUSTERS + 1
+LAYER_SHAPES = [(MLP_HIDDEN, MLP_IN), (MLP_HIDDEN, MLP_HIDDEN), (MLP_HIDDEN, MLP_HIDDEN), (MLP_OUT, MLP_HIDDEN)]
+STAT.update(mlp_ms=14, seeded_ms=15, train_ms=16, train_loss=17, clusters=18, seed_top1=19, seed_top4=20)
+
+
+def _bind_learned(L):
+    if getattr(L, "_learned_bound", False):
+        return L
+    vp, u64, ci = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+    L.prt_mi355x_learned_setup.argtypes = [vp, u64]
+    L.prt_mi355x_learned_train.argtypes = [vp, vp, u64, ci, ci, ctypes.c_float]
+    L.prt_mi355x_nearest_hits_learned_device.argtypes = [vp, vp, u64, ctypes.c_uint32, vp, ctypes.c_uint32, vp, vp]
+    L.prt_mi355x_learned_eval.argtypes = [vp, vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.prt_mi355x_learned_clusters.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), vp]
+    L.prt_mi355x_mlp_params.argtypes = [vp, ci, ci, vp, vp]
+    L.prt_mi355x_mlp_grad.argtypes = [vp, vp, u64, vp, ctypes.POINTER(ctypes.c_float), vp]
+    L.prt_mi355x_mlp_grads.argtypes = [vp, ci, vp, vp]
+    L.prt_mi355x_mlp_top4.argtypes = [vp, vp, u64, vp, vp]
+    L._learned_bound = True
+    return L
+
+
+def _learned_lib():
+    return _bind_learned(lib())
+
+
+def learned_setup(ctx, seed=1):
+    ctx._check(_learned_lib().prt_mi355x_learned_setup(ctx.h, seed))
+
+
+def learned_train(ctx, rays_ptr, n, epochs=1, batch=8192, lr=1e-3):
+    ctx._check(_learned_lib().prt_mi355x_learned_train(ctx.h, ctypes.c_void_p(rays_ptr), n, epochs, batch, lr))
+
+
+def nearest_hits_learned_device(ctx, rays_ptr, n, mask, out_ptr, stream=None):
+    size, off = layout(mask)
+    offs = np.asarray(off, np.int32)
+    ctx._check(_learned_lib().prt_mi355x_nearest_hits_learned_device(
+        ctx.h, ctypes.c_void_p(rays_ptr), n, mask, ctypes.c_void_p(out_ptr), size, _p(offs), ctypes.c_void_p(stream or 0)))
+
+
+def learned_eval(ctx, rays_ptr, n):
+    t1, t4 = ctypes.c_double(), ctypes.c_double()
+    ctx._check(_learned_lib().prt_mi355x_learned_eval(ctx.h, ctypes.c_void_p(rays_ptr), n, ctypes.byref(t1),
+                                                      ctypes.byref(t4)))
+    return t1.value, t4.value
+
+
+def learned_clusters(ctx, n_tris):
+    count = ctypes.c_uint32()
+    tc = np.zeros(n_tris, np.uint32)
+    ctx._check(_learned_lib().prt_mi355x_learned_clusters(ctx.h, ctypes.byref(count), _p(tc)))
+    return count.value, tc
+
+
+def mlp_params(ctx, layer):
+    o, i = LAYER_SHAPES[layer]
+    w, b = np.zeros((o, i), np.float32), np.zeros(o, np.float32)
+    ctx._check(_learned_lib().prt_mi355x_mlp_params(ctx.h, layer, 1, _p(w), _p(b)))
+    return w, b
+
+
+def set_mlp_params(ctx, layer, w, b):
+    w = np.ascontiguousarray(w, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    ctx._check(_learned_lib().prt_mi355x_mlp_params(ctx.h, layer, 0, _p(w), _p(b)))
+
+
+def mlp_grad(ctx, rays, labels):
+    """Loss, encoded inputs (uint16 bf16 bits [n][192]) and per-layer (gW^T, gb) of one batch."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    labels = np.ascontiguousarray(labels, np.uint32)
+    n = len(rays)
+    loss = ctypes.c_float()
+    x0 = np.zeros((n, MLP_IN), np.uint16)
+    ctx._check(_learned_lib().prt_mi355x_mlp_grad(ctx.h, _p(rays), n, _p(labels), ctypes.byref(loss), _p(x0)))
+    grads = []
+    for layer in range(4):
+        o, i = LAYER_SHAPES[layer]
+        gw, gb = np.zeros((o, i), np.float32), np.zeros(o, np.float32)
+        ctx._check(_learned_lib().prt_mi355x_mlp_grads(ctx.h, layer, _p(gw), _p(gb)))
+        grads.append((gw, gb))
+    return loss.value, x0, grads
+
+
+def mlp_top4(ctx, rays, with_logits=False):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    top = np.zeros((n, 4), np.uint32)
+    logits = np.zeros((n, MLP_OUT), np.float32) if with_logits else None
+    ctx._check(_learned_lib().prt_mi355x_mlp_top4(ctx.h, _p(rays), n, _p(top),
+                                                   _p(logits) if with_logits else None))
+    return (top, logits) if with_logits else top

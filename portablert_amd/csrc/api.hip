@@ -10,38 +10,13 @@
 
 #include "../../include/prt_mi355x.h"
 #include "common.h"
+#include "ctx.h"
 #include "internal.h"
 
-struct prt_ctx {
-	int device = 0;
-	hipStream_t stream = nullptr;
-	std::string name;
-	std::string err;
-	prt::Bvh8 bvh;
-	prt::BuildScratch bs;
-	prt::BuildParams bp;
-	prt::TraceParams tp;
-	prt::TraceScratch ts;
-	prt::DevBuf rays_stage, out_stage;
-	bool has_scene = false;
-	int time_kernel = 0;
-	int margin_log2 = 10;
-	int64_t alloc_limit = 0;
-	double build_ms = 0.0, trace_ms = 0.0, api_ms = 0.0;
-	prt::TraceStats stats;
-};
+using prt::fail;
+using prt::hip_fail;
 
 namespace {
-
-int fail(prt_ctx *c, int code, const std::string &msg) {
-	if (c) c->err = msg;
-	return code;
-}
-
-int hip_fail(prt_ctx *c, hipError_t e, const char *what) {
-	const int code = (e == hipErrorOutOfMemory) ? PRT_ERR_OOM : PRT_ERR_HIP;
-	return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
-}
 
 bool is_gfx950(int dev, std::string *name) {
 	hipDeviceProp_t prop;
@@ -89,6 +64,7 @@ int do_build(prt_ctx *c, uint64_t n) {
 	}
 	c->build_ms = ms;
 	c->has_scene = true;
+	++c->scene_gen;
 	apply_margin(c);
 	return PRT_OK;
 }
@@ -354,6 +330,13 @@ int prt_mi355x_get_stat(const prt_ctx *c, int stat, double *v) {
 	case PRT_STAT_NODE_PHASES: *v = static_cast<double>(c->stats.node_phases); break;
 	case PRT_STAT_TRI_PHASES: *v = static_cast<double>(c->stats.tri_phases); break;
 	case PRT_STAT_CULLED_POPS: *v = static_cast<double>(c->stats.culled_pops); break;
+	case PRT_STAT_MLP_MS: *v = c->mlp_ms; break;
+	case PRT_STAT_SEEDED_MS: *v = c->seeded_ms; break;
+	case PRT_STAT_TRAIN_MS: *v = c->train_ms; break;
+	case PRT_STAT_TRAIN_LOSS: *v = c->train_loss; break;
+	case PRT_STAT_CLUSTERS: *v = c->clusters.count; break;
+	case PRT_STAT_SEED_TOP1: *v = c->seed_top1; break;
+	case PRT_STAT_SEED_TOP4: *v = c->seed_top4; break;
 	default: return PRT_ERR_ARG;
 	}
 	return PRT_OK;

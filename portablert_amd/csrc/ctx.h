@@ -1,0 +1,51 @@
+// The C ABI context (prt_ctx) shared by api.hip and learned.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/prt_mi355x.h"
+#include "internal.h"
+#include "mlp.h"
+
+struct prt_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string name;
+	std::string err;
+	prt::Bvh8 bvh;
+	prt::BuildScratch bs;
+	prt::BuildParams bp;
+	prt::TraceParams tp;
+	prt::TraceScratch ts;
+	prt::DevBuf rays_stage, out_stage;
+	bool has_scene = false;
+	int time_kernel = 0;
+	int margin_log2 = 10;
+	int64_t alloc_limit = 0;
+	double build_ms = 0.0, trace_ms = 0.0, api_ms = 0.0;
+	prt::TraceStats stats;
+	// learned seeding
+	prt::Mlp mlp;
+	prt::Clusters clusters;
+	prt::DevBuf top4, labels, hits_tmp;
+	double mlp_ms = 0.0, seeded_ms = 0.0, train_ms = 0.0, train_loss = 0.0;
+	double seed_top1 = 0.0, seed_top4 = 0.0;
+	uint64_t learned_scene = 0;  // set_tris generation the clusters were built for
+	uint64_t scene_gen = 0;
+};
+
+namespace prt {
+
+inline int fail(prt_ctx *c, int code, const std::string &msg) {
+	if (c) c->err = msg;
+	return code;
+}
+
+inline int hip_fail(prt_ctx *c, hipError_t e, const char *what) {
+	const int code = (e == hipErrorOutOfMemory) ? PRT_ERR_OOM : PRT_ERR_HIP;
+	return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace prt

@@ -13,6 +13,10 @@ namespace prt {
 struct DevBuf {
 	void *ptr = nullptr;
 	size_t bytes = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf &) = delete;
+	DevBuf &operator=(const DevBuf &) = delete;
+	~DevBuf() { release(); }
 	hipError_t ensure(size_t want) {
 		if (want <= bytes) return hipSuccess;
 		if (ptr) {
@@ -103,7 +107,17 @@ hipError_t trace_errors(TraceScratch &s, hipStream_t st, uint32_t *flags);
 // Traces n device rays against `b`, writing HitReg records at `out` (device) with the layout
 // of `mask`. Launches on `st`; does not synchronise. `ms` receives the duration of the
 // traversal kernel measured with HIP events when non-null (synchronises then).
+// Per-ray seed clusters for the learned traversal: 4 cluster ids per ray (>= count: none),
+// and each cluster's stack entry and fp32 box.
+struct SeedArgs {
+	const uint32_t *seeds;
+	const uint32_t *entry;
+	const float *box;
+	uint32_t count;
+};
+
 hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
-                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats);
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats,
+                      const SeedArgs *seeds = nullptr);
 
 }  // namespace prt

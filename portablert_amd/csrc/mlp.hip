@@ -1,0 +1,593 @@
+// The learned t_max seeding network: a 192 -> 256 -> 256 -> 256 -> 4097 MLP (ReLU hidden
+// layers, BF16 weights and activations, FP32 accumulation) trained and run with hand-written
+// MFMA kernels on gfx950. It predicts, per ray, the BVH8 cluster (subtree) that holds the hit
+// triangle, or "miss" (logit 4096). No counterpart in the reference.
+//
+// Every matrix product is C[M,N] = A[M,K] * B[N,K]^T with both operands K-contiguous (bf16),
+// computed by k_gemm with v_mfma_f32_16x16x32_bf16 (lane layout checked on the device:
+// A[l%16][(l/16)*8+i], B[(l/16)*8+i][l%16], C[(l/16)*4+i][l%16]). Weights are kept as
+// W^T [out, in] (forward) and W [in, out] (backward data gradient); weight gradients use
+// transposed activations so that the batch is the contiguous reduction dimension.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+#include "mlp.h"
+
+namespace prt {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int kBM = 128, kBN = 128, kBK = 32, kThreads = 256;
+constexpr int kCPad = 132;  // fp32 row stride of the epilogue tile in LDS
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+	// round to nearest even (inputs are finite here)
+	const uint32_t u = __float_as_uint(f);
+	const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+	return static_cast<uint16_t>(r >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+enum Epi : int {
+	E_BIAS_RELU_BF16 = 0,  // out bf16 = relu(acc + bias[n])
+	E_BIAS_F32 = 1,        // out f32 = acc + bias[n]
+	E_MASK_BF16 = 2,       // out bf16 = acc * (aux_bf16[m][n] > 0)
+	E_F32 = 3,             // out f32 = acc (split-K partial: out += split * M * N)
+	E_TOP4 = 4             // per row: top-4 of (acc + bias) over this column tile -> out (val f32, idx u32)
+};
+
+struct GemmArgs {
+	const uint16_t *A;   // [M][lda] bf16
+	const uint16_t *B;   // [N][ldb] bf16
+	void *out;
+	const float *bias;   // [N] (E_BIAS_*, E_TOP4)
+	const uint16_t *aux; // [M][ldo] bf16 (E_MASK_BF16)
+	int M, N, K, lda, ldb, ldo;
+	int k_split;         // K range per z-slice (split-K), multiple of kBK
+	int n_valid;         // columns >= n_valid are padding (E_TOP4 ignores them)
+};
+
+// C tile (128 x 128) of one workgroup: 4 waves, each 64 x 64 as 4 x 4 MFMA 16x16 tiles.
+template <int EPI>
+__global__ __launch_bounds__(kThreads) void k_gemm(GemmArgs g) {
+	__shared__ __align__(16) unsigned char smem[kBM * kCPad * 4];
+	uint16_t *As = reinterpret_cast<uint16_t *>(smem);            // [128][32]
+	uint16_t *Bs = As + kBM * kBK;                                // [128][32]
+	float *Cs = reinterpret_cast<float *>(smem);                  // [128][kCPad] after the K loop
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+	const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+	const int kbeg = blockIdx.z * g.k_split;
+	const int kend = min(g.K, kbeg + g.k_split);
+	f32x4 acc[4][4];
+#pragma unroll
+	for (int i = 0; i < 4; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+	for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+		// global -> LDS: 512 chunks of 16 bytes per operand, two per thread
+#pragma unroll
+		for (int c2 = 0; c2 < 2; ++c2) {
+			const int c = tid + c2 * kThreads;
+			const int row = c >> 2, kq = (c & 3) * 8;
+			uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
+			if (m0 + row < g.M) va = *reinterpret_cast<const uint4 *>(g.A + static_cast<size_t>(m0 + row) * g.lda + k0 + kq);
+			if (n0 + row < g.N) vb = *reinterpret_cast<const uint4 *>(g.B + static_cast<size_t>(n0 + row) * g.ldb + k0 + kq);
+			*reinterpret_cast<uint4 *>(As + row * kBK + kq) = va;
+			*reinterpret_cast<uint4 *>(Bs + row * kBK + kq) = vb;
+		}
+		__syncthreads();
+		bf16x8 af[4], bfr[4];
+#pragma unroll
+		for (int s = 0; s < 4; ++s) {
+			af[s] = *reinterpret_cast<const bf16x8 *>(As + (wr + s * 16 + (lane & 15)) * kBK + (lane >> 4) * 8);
+			bfr[s] = *reinterpret_cast<const bf16x8 *>(Bs + (wc + s * 16 + (lane & 15)) * kBK + (lane >> 4) * 8);
+		}
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+		__syncthreads();
+	}
+	// accumulators -> LDS tile (row-major fp32)
+#pragma unroll
+	for (int i = 0; i < 4; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+#pragma unroll
+			for (int e = 0; e < 4; ++e)
+				Cs[(wr + i * 16 + (lane >> 4) * 4 + e) * kCPad + wc + j * 16 + (lane & 15)] = acc[i][j][e];
+	__syncthreads();
+
+	if constexpr (EPI == E_TOP4) {
+		// two threads per row, 64 columns each, then a pairwise merge
+		const int row = tid >> 1, half = tid & 1;
+		float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+		uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+		for (int c = half * 64; c < half * 64 + 64; ++c) {
+			const int n = n0 + c;
+			if (n >= g.n_valid) break;
+			const float x = Cs[row * kCPad + c] + g.bias[n];
+			if (x > v[3]) {  // insert, keeping v sorted descending (ties keep the lower index first)
+				v[3] = x;
+				ix[3] = static_cast<uint32_t>(n);
+#pragma unroll
+				for (int q = 3; q > 0; --q) {
+					if (v[q] > v[q - 1]) {
+						const float tv = v[q]; v[q] = v[q - 1]; v[q - 1] = tv;
+						const uint32_t ti = ix[q]; ix[q] = ix[q - 1]; ix[q - 1] = ti;
+					}
+				}
+			}
+		}
+		float ov[4];
+		uint32_t oi[4];
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			ov[q] = __shfl_xor(v[q], 1);
+			oi[q] = __shfl_xor(ix[q], 1);
+		}
+		if (half == 0 && m0 + row < g.M) {
+			// merge two descending lists (the partner holds the higher column half)
+			float rv[4];
+			uint32_t ri[4];
+			int a = 0, b = 0;
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				const bool take_a = v[a] >= ov[b];
+				rv[q] = take_a ? v[a] : ov[b];
+				ri[q] = take_a ? ix[a] : oi[b];
+				a += take_a ? 1 : 0;
+				b += take_a ? 0 : 1;
+			}
+			float2 *o = reinterpret_cast<float2 *>(g.out) + (static_cast<size_t>(m0 + row) * gridDim.x + blockIdx.x) * 4;
+#pragma unroll
+			for (int q = 0; q < 4; ++q) o[q] = make_float2(rv[q], __uint_as_float(ri[q]));
+		}
+		return;
+	}
+	// element-wise epilogues: each thread handles 64 consecutive columns... of half a row
+	const int row = tid >> 1, cbase = (tid & 1) * 64;
+	const int m = m0 + row;
+	if (m >= g.M) return;
+	for (int c = cbase; c < cbase + 64; ++c) {
+		const int n = n0 + c;
+		if (n >= g.N) break;
+		const float x = Cs[row * kCPad + c];
+		const size_t o = static_cast<size_t>(m) * g.ldo + n;
+		if constexpr (EPI == E_BIAS_RELU_BF16) {
+			reinterpret_cast<uint16_t *>(g.out)[o] = f2bf(fmaxf(x + g.bias[n], 0.0f));
+		} else if constexpr (EPI == E_BIAS_F32) {
+			reinterpret_cast<float *>(g.out)[o] = x + g.bias[n];
+		} else if constexpr (EPI == E_MASK_BF16) {
+			reinterpret_cast<uint16_t *>(g.out)[o] = bf2f(g.aux[o]) > 0.0f ? f2bf(x) : static_cast<uint16_t>(0);
+		} else if constexpr (EPI == E_F32) {
+			reinterpret_cast<float *>(g.out)[static_cast<size_t>(blockIdx.z) * g.M * g.ldo + o] = x;
+		}
+	}
+}
+
+// Positional encoding: 6 inputs (origin scaled to [-1,1] by the scene box, direction) times
+// 16 frequencies 2^(j/2) * pi, sin and cos: 192 features, written as bf16 [n][192] in the
+// order (input, frequency, sin/cos).
+__global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, uint64_t first, uint32_t n, float4 lo,
+                                                float4 inv_ext, uint16_t *__restrict__ x) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	const float *r = rays + 6 * (first + i);
+	float in[6];
+	in[0] = (r[0] - lo.x) * inv_ext.x * 2.0f - 1.0f;
+	in[1] = (r[1] - lo.y) * inv_ext.y * 2.0f - 1.0f;
+	in[2] = (r[2] - lo.z) * inv_ext.z * 2.0f - 1.0f;
+	in[3] = r[3];
+	in[4] = r[4];
+	in[5] = r[5];
+	uint16_t *o = x + static_cast<size_t>(i) * kMlpIn;
+#pragma unroll
+	for (int a = 0; a < 6; ++a) {
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			// v_sin/v_cos take revolutions: sin(2 pi y); y = in * 2^(j/2) / 2
+			const float y = in[a] * (0.5f * exp2f(0.5f * static_cast<float>(j)));
+			o[a * 32 + j * 2] = f2bf(__builtin_amdgcn_sinf(y));
+			o[a * 32 + j * 2 + 1] = f2bf(__builtin_amdgcn_cosf(y));
+		}
+	}
+}
+
+// Softmax cross-entropy over the first n_valid logits of each row. Writes dlogits =
+// (softmax - onehot) / batch as bf16 (0 in padding columns) and accumulates the mean loss.
+__global__ __launch_bounds__(256) void k_softmax_ce(const float *__restrict__ logits, const uint32_t *__restrict__ label,
+                                                    int rows, int ld, int n_valid, float inv_batch,
+                                                    uint16_t *__restrict__ dlog, float *loss) {
+	const int row = blockIdx.x;
+	if (row >= rows) return;
+	__shared__ float red[256];
+	const float *L = logits + static_cast<size_t>(row) * ld;
+	float mx = -__builtin_inff();
+	for (int c = threadIdx.x; c < n_valid; c += 256) mx = fmaxf(mx, L[c]);
+	red[threadIdx.x] = mx;
+	__syncthreads();
+	for (int s = 128; s > 0; s >>= 1) {
+		if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+		__syncthreads();
+	}
+	mx = red[0];
+	__syncthreads();
+	float sum = 0.0f;
+	for (int c = threadIdx.x; c < n_valid; c += 256) sum += expf(L[c] - mx);
+	red[threadIdx.x] = sum;
+	__syncthreads();
+	for (int s = 128; s > 0; s >>= 1) {
+		if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+		__syncthreads();
+	}
+	sum = red[0];
+	const uint32_t y = label[row];
+	const float inv = 1.0f / sum;
+	uint16_t *D = dlog + static_cast<size_t>(row) * ld;
+	for (int c = threadIdx.x; c < ld; c += 256) {
+		float gval = 0.0f;
+		if (c < n_valid) gval = (expf(L[c] - mx) * inv - (static_cast<uint32_t>(c) == y ? 1.0f : 0.0f)) * inv_batch;
+		D[c] = f2bf(gval);
+	}
+	if (threadIdx.x == 0) atomicAdd(loss, (logf(sum) + mx - L[y]) * inv_batch);
+}
+
+// out[c][r] = in[r][c] for a bf16 [rows][cols] matrix (32 x 32 tiles through LDS).
+__global__ __launch_bounds__(256) void k_transpose(const uint16_t *__restrict__ in, int rows, int cols, int ld_in,
+                                                   uint16_t *__restrict__ out, int ld_out) {
+	__shared__ uint16_t t[32][33];
+	const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for (int j = ty; j < 32; j += 8) {
+		const int r = by + j, c = bx + tx;
+		t[j][tx] = (r < rows && c < cols) ? in[static_cast<size_t>(r) * ld_in + c] : static_cast<uint16_t>(0);
+	}
+	__syncthreads();
+	for (int j = ty; j < 32; j += 8) {
+		const int c = bx + j, r = by + tx;
+		if (c < cols && r < rows) out[static_cast<size_t>(c) * ld_out + r] = t[tx][j];
+	}
+}
+
+// Sums split-K partials: g[i] = sum_s part[s][i].
+__global__ __launch_bounds__(256) void k_reduce_split(const float *__restrict__ part, int splits, size_t n,
+                                                      float *__restrict__ g) {
+	const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= n) return;
+	float s = 0.0f;
+	for (int k = 0; k < splits; ++k) s += part[k * n + i];
+	g[i] = s;
+}
+
+// Column sums of a bf16 [rows][ld] matrix over its first cols columns (bias gradient).
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ d, int rows, int cols, int ld,
+                                                float *__restrict__ g) {
+	const int c = blockIdx.x * 256 + threadIdx.x;
+	if (c >= cols) return;
+	float s = 0.0f;
+	for (int r = 0; r < rows; ++r) s += bf2f(d[static_cast<size_t>(r) * ld + c]);
+	g[c] = s;
+}
+
+// Adam on fp32 master parameters; refreshes the bf16 copy (W^T [rows][cols]) and, when wt is
+// non-null, the transposed bf16 copy W [cols][rows].
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                                              float *__restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2, uint16_t *__restrict__ bf, uint16_t *__restrict__ bft,
+                                              int rows, int cols) {
+	const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= n) return;
+	const float gi = g[i];
+	const float mi = b1 * m[i] + (1.0f - b1) * gi;
+	const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+	m[i] = mi;
+	v[i] = vi;
+	const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+	p[i] = pi;
+	if (bf) {
+		const uint16_t h = f2bf(pi);
+		bf[i] = h;
+		if (bft) {
+			const int r = static_cast<int>(i / cols), c = static_cast<int>(i % cols);
+			bft[static_cast<size_t>(c) * rows + r] = h;
+		}
+	}
+}
+
+// Merges per-tile top-4 lists into the top-4 clusters of each ray.
+__global__ __launch_bounds__(256) void k_top4_merge(const float2 *__restrict__ part, int rows, int tiles,
+                                                    uint32_t *__restrict__ top) {
+	const int r = blockIdx.x * 256 + threadIdx.x;
+	if (r >= rows) return;
+	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+	const float2 *p = part + static_cast<size_t>(r) * tiles * 4;
+	for (int k = 0; k < tiles * 4; ++k) {
+		const float x = p[k].x;
+		const uint32_t id = __float_as_uint(p[k].y);
+		if (x > v[3] || (x == v[3] && id < ix[3])) {
+			v[3] = x;
+			ix[3] = id;
+#pragma unroll
+			for (int q = 3; q > 0; --q) {
+				if (v[q] > v[q - 1] || (v[q] == v[q - 1] && ix[q] < ix[q - 1])) {
+					const float tv = v[q]; v[q] = v[q - 1]; v[q - 1] = tv;
+					const uint32_t ti = ix[q]; ix[q] = ix[q - 1]; ix[q - 1] = ti;
+				}
+			}
+		}
+	}
+	uint4 o = make_uint4(ix[0], ix[1], ix[2], ix[3]);
+	reinterpret_cast<uint4 *>(top)[r] = o;
+}
+
+// Deterministic initialisation (uniform +-sqrt(6/(fan_in+fan_out)) from a counter hash).
+__global__ __launch_bounds__(256) void k_init(float *__restrict__ w, size_t n, int fan_in, int fan_out, uint64_t seed,
+                                              int rows_valid, int cols) {
+	const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= n) return;
+	uint64_t z = seed + i * 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	z ^= z >> 31;
+	const float u = static_cast<float>(z >> 40) * 0x1p-24f * 2.0f - 1.0f;
+	const float a = sqrtf(6.0f / static_cast<float>(fan_in + fan_out));
+	w[i] = (static_cast<int>(i / cols) < rows_valid) ? u * a : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_fill_f32(float *p, size_t n, float v) {
+	const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ p, size_t n, uint16_t *__restrict__ bf,
+                                                 uint16_t *__restrict__ bft, int rows, int cols) {
+	const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+	if (i >= n) return;
+	const uint16_t h = f2bf(p[i]);
+	bf[i] = h;
+	if (bft) bft[static_cast<size_t>(i % cols) * rows + i / cols] = h;
+}
+
+inline unsigned blocks(size_t n, int per = 256) { return static_cast<unsigned>((n + per - 1) / per); }
+
+template <int EPI>
+hipError_t gemm(const GemmArgs &g, int splits, hipStream_t st) {
+	dim3 grid((g.N + kBN - 1) / kBN, (g.M + kBM - 1) / kBM, splits);
+	hipLaunchKernelGGL((k_gemm<EPI>), grid, dim3(kThreads), 0, st, g);
+	return hipGetLastError();
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------ host side
+
+const int kLayerIn[4] = {kMlpIn, kMlpHidden, kMlpHidden, kMlpHidden};
+const int kLayerOut[4] = {kMlpHidden, kMlpHidden, kMlpHidden, kMlpOutPad};
+
+hipError_t mlp_init(Mlp &m, uint64_t seed, hipStream_t st) {
+#define PRT_TRY(x)                                                                                 \
+	do {                                                                                           \
+		hipError_t e_ = (x);                                                                       \
+		if (e_ != hipSuccess) return e_;                                                           \
+	} while (0)
+	for (int l = 0; l < 4; ++l) {
+		const size_t n = static_cast<size_t>(kLayerOut[l]) * kLayerIn[l];
+		PRT_TRY(m.w[l].ensure(n * 4));
+		PRT_TRY(m.b[l].ensure(kLayerOut[l] * 4));
+		PRT_TRY(m.wt_bf[l].ensure(n * 2));
+		PRT_TRY(m.w_bf[l].ensure(n * 2));
+		PRT_TRY(m.mw[l].ensure(n * 4));
+		PRT_TRY(m.vw[l].ensure(n * 4));
+		PRT_TRY(m.mb[l].ensure(kLayerOut[l] * 4));
+		PRT_TRY(m.vb[l].ensure(kLayerOut[l] * 4));
+		PRT_TRY(m.gw[l].ensure(n * 4));
+		PRT_TRY(m.gb[l].ensure(kLayerOut[l] * 4));
+		const int valid = l == 3 ? kMlpOut : kLayerOut[l];
+		hipLaunchKernelGGL(k_init, dim3(blocks(n)), dim3(256), 0, st, m.w[l].as<float>(), n, kLayerIn[l], valid,
+		                   seed * 31 + l, valid, kLayerIn[l]);
+		hipLaunchKernelGGL(k_fill_f32, dim3(blocks(kLayerOut[l])), dim3(256), 0, st, m.b[l].as<float>(),
+		                   static_cast<size_t>(kLayerOut[l]), 0.0f);
+		if (l == 3) {  // padding logits never win and get no probability mass
+			hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, st, m.b[l].as<float>() + kMlpOut,
+			                   static_cast<size_t>(kMlpOutPad - kMlpOut), -1e30f);
+		}
+		hipLaunchKernelGGL(k_fill_f32, dim3(blocks(n)), dim3(256), 0, st, m.mw[l].as<float>(), n, 0.0f);
+		hipLaunchKernelGGL(k_fill_f32, dim3(blocks(n)), dim3(256), 0, st, m.vw[l].as<float>(), n, 0.0f);
+		hipLaunchKernelGGL(k_fill_f32, dim3(blocks(kLayerOut[l])), dim3(256), 0, st, m.mb[l].as<float>(),
+		                   static_cast<size_t>(kLayerOut[l]), 0.0f);
+		hipLaunchKernelGGL(k_fill_f32, dim3(blocks(kLayerOut[l])), dim3(256), 0, st, m.vb[l].as<float>(),
+		                   static_cast<size_t>(kLayerOut[l]), 0.0f);
+		hipLaunchKernelGGL(k_to_bf16, dim3(blocks(n)), dim3(256), 0, st, m.w[l].as<float>(), n, m.wt_bf[l].as<uint16_t>(),
+		                   m.w_bf[l].as<uint16_t>(), kLayerOut[l], kLayerIn[l]);
+	}
+	m.step = 0;
+	m.ready = true;
+	return hipGetLastError();
+}
+
+hipError_t mlp_refresh_bf16(Mlp &m, hipStream_t st) {
+	for (int l = 0; l < 4; ++l) {
+		const size_t n = static_cast<size_t>(kLayerOut[l]) * kLayerIn[l];
+		hipLaunchKernelGGL(k_to_bf16, dim3(blocks(n)), dim3(256), 0, st, m.w[l].as<float>(), n, m.wt_bf[l].as<uint16_t>(),
+		                   m.w_bf[l].as<uint16_t>(), kLayerOut[l], kLayerIn[l]);
+	}
+	return hipGetLastError();
+}
+
+static hipError_t ensure_act(Mlp &m, int rows, bool train) {
+	PRT_TRY(m.x0.ensure(static_cast<size_t>(rows) * kMlpIn * 2));
+	for (int l = 0; l < 3; ++l) PRT_TRY(m.h[l].ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
+	if (train) {
+		PRT_TRY(m.logits.ensure(static_cast<size_t>(rows) * kMlpOutPad * 4));
+		PRT_TRY(m.dlog.ensure(static_cast<size_t>(rows) * kMlpOutPad * 2));
+		PRT_TRY(m.dlog_t.ensure(static_cast<size_t>(rows) * kMlpOutPad * 2));
+		for (int l = 0; l < 3; ++l) PRT_TRY(m.dh[l].ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
+		PRT_TRY(m.act_t.ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
+		PRT_TRY(m.dact_t.ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
+		PRT_TRY(m.split.ensure(static_cast<size_t>(8) * kMlpOutPad * kMlpHidden * 4));
+		PRT_TRY(m.loss.ensure(4));
+	} else {
+		const int tiles = kMlpOutPad / kBN;
+		PRT_TRY(m.top_part.ensure(static_cast<size_t>(rows) * tiles * 4 * 8));
+	}
+	return hipSuccess;
+}
+
+hipError_t mlp_encode(Mlp &m, const float *rays, uint64_t first, int rows, hipStream_t st) {
+	// the encoding may be the first kernel of a batch: size its output here
+	PRT_TRY(m.x0.ensure(static_cast<size_t>(rows) * kMlpIn * 2));
+	const float4 lo = make_float4(m.scene_lo[0], m.scene_lo[1], m.scene_lo[2], 0.f);
+	const float4 ie = make_float4(m.scene_inv_ext[0], m.scene_inv_ext[1], m.scene_inv_ext[2], 0.f);
+	hipLaunchKernelGGL(k_encode, dim3(blocks(rows)), dim3(256), 0, st, rays, first, static_cast<uint32_t>(rows), lo, ie,
+	                   m.x0.as<uint16_t>());
+	return hipGetLastError();
+}
+
+// Hidden layers: x0 -> h0 -> h1 -> h2.
+static hipError_t forward_hidden(Mlp &m, int rows, hipStream_t st) {
+	const uint16_t *in = m.x0.as<uint16_t>();
+	for (int l = 0; l < 3; ++l) {
+		GemmArgs g{};
+		g.A = in;
+		g.B = m.wt_bf[l].as<uint16_t>();
+		g.out = m.h[l].ptr;
+		g.bias = m.b[l].as<float>();
+		g.M = rows;
+		g.N = kMlpHidden;
+		g.K = kLayerIn[l];
+		g.lda = kLayerIn[l];
+		g.ldb = kLayerIn[l];
+		g.ldo = kMlpHidden;
+		g.k_split = g.K;
+		PRT_TRY(gemm<E_BIAS_RELU_BF16>(g, 1, st));
+		in = m.h[l].as<uint16_t>();
+	}
+	return hipSuccess;
+}
+
+hipError_t mlp_forward_logits(Mlp &m, int rows, hipStream_t st) {
+	PRT_TRY(ensure_act(m, rows, true));
+	PRT_TRY(forward_hidden(m, rows, st));
+	GemmArgs g{};
+	g.A = m.h[2].as<uint16_t>();
+	g.B = m.wt_bf[3].as<uint16_t>();
+	g.out = m.logits.ptr;
+	g.bias = m.b[3].as<float>();
+	g.M = rows;
+	g.N = kMlpOutPad;
+	g.K = kMlpHidden;
+	g.lda = g.ldb = kMlpHidden;
+	g.ldo = kMlpOutPad;
+	g.k_split = g.K;
+	return gemm<E_BIAS_F32>(g, 1, st);
+}
+
+hipError_t mlp_infer_top4(Mlp &m, const float *rays, uint64_t first, int rows, uint32_t *top4, hipStream_t st) {
+	PRT_TRY(ensure_act(m, rows, false));
+	PRT_TRY(mlp_encode(m, rays, first, rows, st));
+	PRT_TRY(forward_hidden(m, rows, st));
+	GemmArgs g{};
+	g.A = m.h[2].as<uint16_t>();
+	g.B = m.wt_bf[3].as<uint16_t>();
+	g.out = m.top_part.ptr;
+	g.bias = m.b[3].as<float>();
+	g.M = rows;
+	g.N = kMlpOutPad;
+	g.K = kMlpHidden;
+	g.lda = g.ldb = kMlpHidden;
+	g.ldo = kMlpOutPad;
+	g.k_split = g.K;
+	g.n_valid = kMlpOut;
+	PRT_TRY(gemm<E_TOP4>(g, 1, st));
+	hipLaunchKernelGGL(k_top4_merge, dim3(blocks(rows)), dim3(256), 0, st, m.top_part.as<float2>(), rows,
+	                   kMlpOutPad / kBN, top4);
+	return hipGetLastError();
+}
+
+// Backward of one batch whose logits are in m.logits; gradients land in m.gw / m.gb.
+hipError_t mlp_backward(Mlp &m, const uint32_t *labels, int rows, hipStream_t st) {
+	PRT_TRY(ensure_act(m, rows, true));
+	hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(256), 0, st, m.loss.as<float>(), static_cast<size_t>(1), 0.0f);
+	hipLaunchKernelGGL(k_softmax_ce, dim3(rows), dim3(256), 0, st, m.logits.as<float>(), labels, rows, kMlpOutPad, kMlpOut,
+	                   1.0f / static_cast<float>(rows), m.dlog.as<uint16_t>(), m.loss.as<float>());
+	const uint16_t *dy = m.dlog.as<uint16_t>();
+	int dy_cols = kMlpOutPad;
+	for (int l = 3; l >= 0; --l) {
+		const int in = kLayerIn[l], out = kLayerOut[l];
+		const uint16_t *x = l == 0 ? m.x0.as<uint16_t>() : m.h[l - 1].as<uint16_t>();
+		// bias gradient: column sums of dy
+		hipLaunchKernelGGL(k_colsum, dim3(blocks(out)), dim3(256), 0, st, dy, rows, out, dy_cols, m.gb[l].as<float>());
+		// weight gradient gW^T[out][in] = dy^T[out][rows] * x^T[in][rows]^T (split over the batch)
+		uint16_t *dyt = (l == 3) ? m.dlog_t.as<uint16_t>() : m.dact_t.as<uint16_t>();
+		hipLaunchKernelGGL(k_transpose, dim3((out + 31) / 32, (rows + 31) / 32), dim3(256), 0, st, dy, rows, out, dy_cols,
+		                   dyt, rows);
+		hipLaunchKernelGGL(k_transpose, dim3((in + 31) / 32, (rows + 31) / 32), dim3(256), 0, st, x, rows, in, in,
+		                   m.act_t.as<uint16_t>(), rows);
+		const int splits = 8;
+		GemmArgs g{};
+		g.A = dyt;
+		g.B = m.act_t.as<uint16_t>();
+		g.out = m.split.ptr;
+		g.M = out;
+		g.N = in;
+		g.K = rows;
+		g.lda = g.ldb = rows;
+		g.ldo = in;
+		g.k_split = ((rows / splits + kBK - 1) / kBK) * kBK;
+		PRT_TRY(gemm<E_F32>(g, splits, st));
+		const size_t nw = static_cast<size_t>(out) * in;
+		hipLaunchKernelGGL(k_reduce_split, dim3(blocks(nw)), dim3(256), 0, st, m.split.as<float>(), splits, nw,
+		                   m.gw[l].as<float>());
+		if (l == 0) break;
+		// data gradient dh[l-1] = (dy * W) masked by relu'(h[l-1])
+		GemmArgs d{};
+		d.A = dy;
+		d.B = m.w_bf[l].as<uint16_t>();
+		d.out = m.dh[l - 1].ptr;
+		d.aux = m.h[l - 1].as<uint16_t>();
+		d.M = rows;
+		d.N = in;
+		d.K = out;
+		d.lda = dy_cols;
+		d.ldb = out;
+		d.ldo = in;
+		d.k_split = d.K;
+		PRT_TRY(gemm<E_MASK_BF16>(d, 1, st));
+		dy = m.dh[l - 1].as<uint16_t>();
+		dy_cols = in;
+	}
+	return hipGetLastError();
+}
+
+hipError_t mlp_adam(Mlp &m, float lr, hipStream_t st) {
+	++m.step;
+	const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+	const float c1 = 1.0f / (1.0f - std::pow(b1, static_cast<float>(m.step)));
+	const float c2 = 1.0f / (1.0f - std::pow(b2, static_cast<float>(m.step)));
+	for (int l = 0; l < 4; ++l) {
+		const size_t n = static_cast<size_t>(kLayerOut[l]) * kLayerIn[l];
+		hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, st, m.w[l].as<float>(), m.gw[l].as<float>(),
+		                   m.mw[l].as<float>(), m.vw[l].as<float>(), n, lr, b1, b2, eps, c1, c2, m.wt_bf[l].as<uint16_t>(),
+		                   m.w_bf[l].as<uint16_t>(), kLayerOut[l], kLayerIn[l]);
+		const size_t nb = l == 3 ? static_cast<size_t>(kMlpOut) : static_cast<size_t>(kLayerOut[l]);
+		hipLaunchKernelGGL(k_adam, dim3(blocks(nb)), dim3(256), 0, st, m.b[l].as<float>(), m.gb[l].as<float>(),
+		                   m.mb[l].as<float>(), m.vb[l].as<float>(), nb, lr, b1, b2, eps, c1, c2,
+		                   static_cast<uint16_t *>(nullptr), static_cast<uint16_t *>(nullptr), 1, 1);
+	}
+	return hipGetLastError();
+#undef PRT_TRY
+}
+
+}  // namespace prt

@@ -44,6 +44,11 @@ struct KArgs {
 	float margin_abs;
 	int cull;
 	int fast_scene;         // every vertex coordinate below 2^60 in magnitude
+	// learned seeding (null when off): 4 cluster ids per ray, cluster entries and fp32 boxes
+	const uint32_t *seeds;
+	const uint32_t *cl_entry;
+	const float *cl_box;
+	uint32_t n_clusters;
 };
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
@@ -372,6 +377,24 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				cur = a.n_nodes > 0 ? 0u : kInvalid;
 				pend = kInvalid;
 				st.sp = 0;
+				if (a.seeds && a.n_nodes > 0) {
+					// the root goes to the bottom of the stack, the predicted clusters above it
+					// (best on top): their subtrees are intersected exactly first and seed the
+					// culling distance; the root traversal afterwards still covers every triangle
+					st.push(0u, -3.402823466e38f);
+					cur = kInvalid;
+					const uint4 s4 = reinterpret_cast<const uint4 *>(a.seeds)[ray_idx];
+					const uint32_t sc[4] = {s4.w, s4.z, s4.y, s4.x};
+#pragma unroll
+					for (int q = 0; q < 4; ++q) {
+						if (sc[q] < a.n_clusters) {
+							const float *bx = a.cl_box + 6ull * sc[q];
+							const float lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
+							const float tmin = child_slab(r, lo, hi, __builtin_inff());
+							if (!__builtin_isnan(tmin)) st.push(a.cl_entry[sc[q]], fmaxf(tmin, -3.402823466e38f));
+						}
+					}
+				}
 			}
 			const uint64_t took = __popcll(need) < avail ? __popcll(need) : avail;
 			chunk_next += took;
@@ -499,7 +522,7 @@ hipError_t launch_b(const KArgs &a, uint32_t grid, int B, int S, bool count, hip
 }  // namespace
 
 hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
-                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats) {
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats, const SeedArgs *seeds) {
 #define PRT_TRY(x)                                                                                 \
 	do {                                                                                           \
 		hipError_t e_ = (x);                                                                       \
@@ -542,6 +565,10 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.margin_rel = p.margin_rel;
 	a.margin_abs = p.margin_abs;
 	a.cull = p.cull;
+	a.seeds = seeds ? seeds->seeds : nullptr;
+	a.cl_entry = seeds ? seeds->entry : nullptr;
+	a.cl_box = seeds ? seeds->box : nullptr;
+	a.n_clusters = seeds ? seeds->count : 0u;
 	a.fast_scene = 1;
 	for (int k = 0; k < 6; ++k) a.fast_scene &= std::fabs(b.scene_box[k]) < 0x1p60f ? 1 : 0;
 

@@ -1,0 +1,95 @@
+"""The learned t_max seeding path on the GPU: the MFMA forward/backward kernels against the CPU
+oracle (oracle/mlp_ref.py), the fused top-4 against the full logits, the cluster cut, and the
+seeded traversal returning exactly the plain traversal's records."""
+import numpy as np
+import pytest
+
+import oracle
+import portablert_amd as prt
+from oracle import mlp_ref
+from portablert_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def room(gpu_ctx):
+    tris = scenes.scene(2)
+    gpu_ctx.set_tris(tris)
+    prt.learned_setup(gpu_ctx, 7)
+    return tris
+
+
+def test_clusters_partition_the_triangles(gpu_ctx, room):
+    count, tc = prt.learned_clusters(gpu_ctx, len(room))
+    assert 3000 < count <= prt.CLUSTERS
+    assert (tc < count).all()
+    sizes = np.bincount(tc, minlength=count)
+    assert (sizes > 0).all() and sizes.sum() == len(room)
+    print(f"{count} clusters, {sizes.min()}..{sizes.max()} triangles", flush=True)
+
+
+def test_gradients_match_oracle(gpu_ctx, room):
+    rng = np.random.default_rng(3)
+    rays = scenes.rays(4, stream=1, first=12345, count=256)
+    labels = rng.integers(0, prt.MLP_OUT, len(rays)).astype(np.uint32)
+    params = [prt.mlp_params(gpu_ctx, layer) for layer in range(4)]
+    loss, x0_bits, grads = prt.mlp_grad(gpu_ctx, rays, labels)
+    x0 = mlp_ref.bf16_bits_to_f64(x0_bits)
+    # the encoding itself: sin/cos of the scaled inputs within bf16 resolution
+    assert np.abs(x0).max() <= 1.0
+    want_loss, want = mlp_ref.loss_and_grads(x0, params, labels.astype(np.int64), bf16=True)
+    assert abs(loss - want_loss) <= 1e-3 * abs(want_loss), (loss, want_loss)
+    for layer in range(4):
+        for k, name in ((0, "W"), (1, "b")):
+            g, w = grads[layer][k].astype(np.float64), want[layer][k]
+            scale = np.abs(w).max()
+            err = np.abs(g - w).max()
+            assert err <= 2e-2 * scale + 1e-7, (layer, name, err, scale)
+            cos = float((g * w).sum() / (np.linalg.norm(g) * np.linalg.norm(w) + 1e-30))
+            assert cos > 0.9999, (layer, name, cos)
+
+
+def test_top4_matches_full_logits(gpu_ctx, room):
+    rays = scenes.rays(4, stream=2, first=777, count=512)
+    top, logits = prt.mlp_top4(gpu_ctx, rays, with_logits=True)
+    order = np.argsort(-logits, axis=1, kind="stable")[:, :4]
+    gap = np.sort(logits, axis=1)[:, ::-1]
+    clear = (gap[:, 3] - gap[:, 4]) > 1e-3 * np.abs(gap[:, 3]).clip(1e-3)
+    assert clear.mean() > 0.9
+    assert np.array_equal(top[clear], order[clear].astype(np.uint32))
+
+
+def test_learned_traversal_identical_to_plain(gpu_ctx, room):
+    import torch
+    train = scenes.rays(4, stream=1, count=262144)
+    test = scenes.rays(4, stream=2, count=1 << 20)
+    d_train = torch.from_numpy(train).cuda()
+    d_test = torch.from_numpy(test).cuda()
+    mask = prt.T | prt.PRIMITIVE_ID | prt.UV
+    size, _ = prt.layout(mask)
+    before = prt.learned_eval(gpu_ctx, d_test.data_ptr(), len(test))
+    loss0 = None
+    prt.learned_train(gpu_ctx, d_train.data_ptr(), len(train), epochs=1, batch=8192, lr=2e-3)
+    loss0 = gpu_ctx.stat("train_loss")
+    prt.learned_train(gpu_ctx, d_train.data_ptr(), len(train), epochs=2, batch=8192, lr=1e-3)
+    loss1 = gpu_ctx.stat("train_loss")
+    after = prt.learned_eval(gpu_ctx, d_test.data_ptr(), len(test))
+    print(f"loss {loss0:.3f} -> {loss1:.3f}; top1/top4 before {before} after {after}", flush=True)
+    assert loss1 < np.log(prt.MLP_OUT)
+    assert after[1] > before[1] + 0.05
+    out_plain = torch.zeros(len(test) * size, dtype=torch.uint8, device="cuda")
+    out_learn = torch.zeros_like(out_plain)
+    for fmt in (1, 0):
+        gpu_ctx.set_option("node_format", fmt)
+        gpu_ctx.nearest_hits_device(d_test.data_ptr(), len(test), mask, out_plain.data_ptr())
+        prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), len(test), mask, out_learn.data_ptr())
+        gpu_ctx.synchronize()
+        assert torch.equal(out_plain, out_learn), f"learned differs from plain (node format {fmt})"
+    gpu_ctx.set_option("node_format", 1)
+    got = out_learn.cpu().numpy().view(prt.hit_dtype(mask))
+    idx = np.random.default_rng(0).choice(len(test), 8192, replace=False)
+    want = oracle.trace(room, test[idx], oracle.LOWEST_ID)
+    assert (got["t"][idx].view(np.uint32) == want["t"].view(np.uint32)).all()
+    hit = want["t"] < np.inf
+    assert (got["primitive_id"][idx][hit] == want["primitive_id"][hit]).all()
