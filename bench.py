@@ -51,6 +51,85 @@ def cpu_baseline(tris, rays, target_s=10.0):
                       f"oracle depth-first mode (the reference CPU backend restated), {threads} threads, {dt:.1f} s"}
 
 
+def learned_leg(ctx, tris, steps, warmup, epochs):
+    """Learned t_max seeding on the room scene: train on 16,777,216 rays traced by the plain
+    method, then time plain and learned-seeded traversal on 8,388,608 fresh rays (same records,
+    same rays). Training time is reported, not part of the traversal rate."""
+    import torch
+
+    import portablert_amd as prt
+    from portablert_amd import scenes
+    train = scenes.rays(4, stream=1)
+    test = scenes.rays(4, stream=2)
+    n = len(test)
+    d_train = torch.from_numpy(train).cuda()
+    d_test = torch.from_numpy(test).cuda()
+    torch.cuda.synchronize()  # the library trains on its own stream
+    del train
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    out_p = torch.empty(n * size, dtype=torch.uint8, device="cuda")
+    out_l = torch.empty_like(out_p)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    prt.learned_setup(ctx, 2024)
+    untrained = prt.learned_eval(ctx, d_test.data_ptr(), n)
+    t0 = time.perf_counter()
+    prt.learned_train(ctx, d_train.data_ptr(), len(d_train), epochs=epochs, batch=8192, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    top1, top4 = prt.learned_eval(ctx, d_test.data_ptr(), n)
+    log(f"learned: trained {epochs} epoch(s) on {len(d_train)} rays in {train_s:.1f} s, loss "
+        f"{ctx.stat('train_loss'):.3f}; top-1 {top1:.3f} top-4 {top4:.3f} (untrained {untrained[1]:.4f})")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    plain_s = timed(lambda: ctx.nearest_hits_device(d_test.data_ptr(), n, mask, out_p.data_ptr(), sp))
+    learn_s = timed(lambda: prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp))
+    identical = bool(torch.equal(out_p, out_l))
+    ctx.set_option("time_kernel", 1)
+    prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp)
+    mlp_ms, seeded_ms = ctx.stat("mlp_ms"), ctx.stat("seeded_ms")
+    ctx.set_option("time_kernel", 0)
+    ctx.set_option("count_stats", 1)
+    ctx.nearest_hits_device(d_test.data_ptr(), n, mask, out_p.data_ptr(), sp)
+    ctx.synchronize()
+    nodes_plain = ctx.stat("node_visits") / n
+    prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp)
+    ctx.synchronize()
+    nodes_seeded = ctx.stat("node_visits") / n
+    ctx.set_option("count_stats", 0)
+    flops = 2 * (192 * 256 + 2 * 256 * 256 + 256 * 4097) * n
+    res = {
+        "rays": "room scene; 16,777,216 training rays, 8,388,608 fresh rays (uniform origins, uniform directions)",
+        "epochs": epochs,
+        "train_s": round(train_s, 2),
+        "train_loss": round(ctx.stat("train_loss"), 4),
+        "plain_mrays_s": round(n / plain_s / 1e6, 2),
+        "learned_mrays_s": round(n / learn_s / 1e6, 2),
+        "speedup": round(plain_s / learn_s, 4),
+        "identical_records": identical,
+        "mlp_ms": round(mlp_ms, 3),
+        "seeded_traversal_ms": round(seeded_ms, 3),
+        "mlp_tflops": round(flops / (mlp_ms * 1e-3) / 1e12, 1) if mlp_ms > 0 else None,
+        "seed_top1": round(top1, 4),
+        "seed_top4": round(top4, 4),
+        "seed_top4_untrained": round(untrained[1], 5),
+        "node_visits_per_ray_plain": round(nodes_plain, 2),
+        "node_visits_per_ray_seeded": round(nodes_seeded, 2),
+    }
+    del d_train
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +140,8 @@ def main():
     ap.add_argument("--cull", type=int, default=1)
     ap.add_argument("--calib", action="store_true",
                     help="also run a 512 MiB elementwise copy kernel (HBM counter calibration)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding leg")
+    ap.add_argument("--epochs", type=int, default=2, help="training epochs of the learned leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU)",
@@ -176,6 +257,10 @@ def main():
         "learned_mrays_s": None,
         "oracle_agreement_sample": agree,
     }
+    if not args.no_learned:
+        lr = learned_leg(ctx, tris, max(3, args.steps // 4), 1, args.epochs)
+        out["learned_mrays_s"] = lr["learned_mrays_s"]
+        out["learned"] = lr
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tris, rays)
     print(json.dumps(out), flush=True)

@@ -267,7 +267,7 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		c->tp.waves_per_cu = static_cast<int>(v);
 		break;
 	case PRT_OPT_LDS_STACK:
-		if (v != 4 && v != 8 && v != 16) return bad();
+		if (v != 8 && v != 16) return bad();
 		c->tp.lds_stack = static_cast<int>(v);
 		break;
 	case PRT_OPT_CHUNK:

@@ -334,7 +334,7 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 	}
 }
 
-template <class Node, int S, int B, bool COUNT>
+template <class Node, int S, int B, bool COUNT, bool SEEDED>
 __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 	__shared__ uint2 lds[S * B];
 	const Node *nodes = static_cast<const Node *>(a.nodes);
@@ -350,6 +350,11 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 	bool active = false;       // lane holds a ray
 	uint32_t cur = kInvalid;   // next entry to process (node or leaf)
 	uint32_t pend = kInvalid;  // parked leaf, tested in the next triangle phase
+	// learned seeding: the seed cluster entries, which of them were searched, and whether the
+	// root (pushed below the seeds) has been reached
+	uint32_t sd[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+	uint32_t seen = 0u;
+	bool root_phase = true;
 	// wave-uniform ray chunk [chunk_next, chunk_end) reserved with one atomic
 	uint64_t chunk_next = 0, chunk_end = 0;
 	bool exhausted = false;
@@ -377,21 +382,28 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				cur = a.n_nodes > 0 ? 0u : kInvalid;
 				pend = kInvalid;
 				st.sp = 0;
-				if (a.seeds && a.n_nodes > 0) {
+				if (SEEDED && a.n_nodes > 0) {
 					// the root goes to the bottom of the stack, the predicted clusters above it
 					// (best on top): their subtrees are intersected exactly first and seed the
-					// culling distance; the root traversal afterwards still covers every triangle
+					// culling distance; the root traversal afterwards covers every triangle
+					// except those of the clusters already searched
 					st.push(0u, -3.402823466e38f);
 					cur = kInvalid;
+					root_phase = false;
+					seen = 0u;
 					const uint4 s4 = reinterpret_cast<const uint4 *>(a.seeds)[ray_idx];
 					const uint32_t sc[4] = {s4.w, s4.z, s4.y, s4.x};
 #pragma unroll
 					for (int q = 0; q < 4; ++q) {
+						sd[q] = kInvalid;
 						if (sc[q] < a.n_clusters) {
 							const float *bx = a.cl_box + 6ull * sc[q];
 							const float lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
 							const float tmin = child_slab(r, lo, hi, __builtin_inff());
-							if (!__builtin_isnan(tmin)) st.push(a.cl_entry[sc[q]], fmaxf(tmin, -3.402823466e38f));
+							if (!__builtin_isnan(tmin)) {
+								sd[q] = a.cl_entry[sc[q]];
+								st.push(sd[q], fmaxf(tmin, -3.402823466e38f));
+							}
 						}
 					}
 				}
@@ -412,8 +424,17 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 			const float limit = cull_limit(r, a);
 			do {
 				const uint2 e = st.pop();
-				if (__uint_as_float(e.y) <= limit) cur = e.x;
-				else if (COUNT) ++c_culled;
+				if (__uint_as_float(e.y) <= limit) {
+					cur = e.x;
+					if (SEEDED && !root_phase) {
+						// a cluster popped here is searched completely before anything below it
+						if (cur == 0u) root_phase = true;
+#pragma unroll
+						for (int q = 0; q < 4; ++q) seen |= (cur == sd[q] && cur != 0u) ? (1u << q) : 0u;
+					}
+				} else if (COUNT) {
+					++c_culled;
+				}
 			} while (cur == kInvalid && st.sp > 0);
 		}
 		if (cur != kInvalid && (cur & kLeafBit) && pend == kInvalid) {
@@ -465,13 +486,22 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 			test_children<Node>(nodes, cur, r, cull_limit(r, a), a.cull && r.fast, h);
 			if (COUNT) ++c_nodes;
 			cur = h.key[0] != kMiss ? h.entry(h.key[0]) : kInvalid;
+			// clusters searched in the seed phase are not searched again
+			auto searched = [&](uint32_t e) {
+				if (!SEEDED || !root_phase) return false;
+				bool s = false;
+#pragma unroll
+				for (int q = 0; q < 4; ++q) s |= ((seen >> q) & 1u) && e == sd[q];
+				return s;
+			};
+			if (searched(cur)) cur = kInvalid;
 			// farthest first, so the nearest remaining child is on top
 #pragma unroll
 			for (int k = 7; k >= 1; --k) {
 				if (h.key[k] != kMiss) {
 					const uint32_t e = h.entry(h.key[k]);
 					if (COUNT) c_leaves += (e & kLeafBit) ? 1u : 0u;
-					st.push(e, key_tmin(h.key[k]));
+					if (!searched(e)) st.push(e, key_tmin(h.key[k]));
 				}
 			}
 		}
@@ -492,17 +522,21 @@ __global__ void k_zero(unsigned long long *p, int n) {
 
 template <class Node, int S, int B>
 hipError_t launch(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
-	if (count)
-		hipLaunchKernelGGL((k_trace<Node, S, B, true>), dim3(grid), dim3(B), 0, st, a);
+	const bool seeded = a.seeds != nullptr;
+	if (count && seeded)
+		hipLaunchKernelGGL((k_trace<Node, S, B, true, true>), dim3(grid), dim3(B), 0, st, a);
+	else if (count)
+		hipLaunchKernelGGL((k_trace<Node, S, B, true, false>), dim3(grid), dim3(B), 0, st, a);
+	else if (seeded)
+		hipLaunchKernelGGL((k_trace<Node, S, B, false, true>), dim3(grid), dim3(B), 0, st, a);
 	else
-		hipLaunchKernelGGL((k_trace<Node, S, B, false>), dim3(grid), dim3(B), 0, st, a);
+		hipLaunchKernelGGL((k_trace<Node, S, B, false, false>), dim3(grid), dim3(B), 0, st, a);
 	return hipGetLastError();
 }
 
 template <class Node, int B>
 hipError_t launch_s(const KArgs &a, uint32_t grid, int S, bool count, hipStream_t st) {
 	switch (S) {
-	case 4: return launch<Node, 4, B>(a, grid, count, st);
 	case 8: return launch<Node, 8, B>(a, grid, count, st);
 	case 16: return launch<Node, 16, B>(a, grid, count, st);
 	default: return hipErrorInvalidValue;
