@@ -160,6 +160,51 @@ int prt_mi355x_mlp_grad(prt_ctx *ctx, const float *rays6, uint64_t n, const uint
 int prt_mi355x_mlp_grads(prt_ctx *ctx, int layer, float *gw, float *gb);
 int prt_mi355x_mlp_top4(prt_ctx *ctx, const float *rays6, uint64_t n, uint32_t *top4, float *logits);
 
+/* ---- Batched multi-scene tracing. No counterpart in the reference, whose backends hold one
+ * scene per set_tris and trace one ray vector per nearest_hits (backend.hpp:19-46): here many
+ * scenes of uneven sizes are built by one set of launches and traced by ONE persistent launch,
+ * capturable in a hipGraph. Triangles of all scenes are concatenated (scene s owns
+ * tri_counts[s] consecutive triangles), and so are rays; each record equals what set_tris(scene
+ * s) + nearest_hits(rays of s) returns (primitive ids are scene-local). Scenes are grouped in 4
+ * size buckets by triangle count (< 4096, < 16384, < 65536, more), each with its own BVH
+ * branching factor, leaf size and work-tile size. */
+
+/* Declares the batch: scene count (1..65536), per-scene triangle and ray counts (at least one
+ * triangle in total), record layout (field mask). Replaces any previous batch; tuned parameters
+ * are kept. */
+int prt_mi355x_batch_setup(prt_ctx *ctx, uint32_t n_scenes, const uint64_t *tri_counts, const uint64_t *ray_counts,
+                           uint32_t mask);
+
+/* Builds every scene and traces every ray, enqueued on `stream` (NULL = the context's stream)
+ * without synchronising. Triangles (9 floats), rays (6 floats) and records in device memory. */
+int prt_mi355x_batch_run(prt_ctx *ctx, const float *d_tris9, const float *d_rays6, void *d_out, void *stream);
+
+/* Captures the same work with these buffers into a hipGraph (after one eager sizing run when
+ * needed); batch_replay launches it. The graph reads the buffers at replay time. */
+int prt_mi355x_batch_capture(prt_ctx *ctx, const float *d_tris9, const float *d_rays6, void *d_out);
+int prt_mi355x_batch_replay(prt_ctx *ctx, void *stream);
+
+/* Waits for the device and reports errors of the last batch run or replay (non-finite vertex:
+ * PRT_ERR_INPUT; BVH deeper than the fixed 48 levels or corrupt: PRT_ERR_HIP). */
+int prt_mi355x_batch_check(prt_ctx *ctx);
+
+/* Parameters: bucket 0..3 -> {width 2..8, leaf 1..4, tile rays 1..2^20}; bucket -1 -> launch
+ * shape {workgroup 64/128/256, waves per CU 1..64, LDS stack 8/16}. Setting drops the graph. */
+int prt_mi355x_batch_get_params(const prt_ctx *ctx, int bucket, int32_t v[3]);
+int prt_mi355x_batch_set_params(prt_ctx *ctx, int bucket, const int32_t v[3]);
+
+/* Autotuner: times candidate parameters on the batch (rebuilding and tracing it, `reps` runs
+ * each, smallest taken) and keeps the fastest; returns whole-batch build + trace time before and
+ * after. Leaves the records of the final run in d_out. Synchronous. */
+int prt_mi355x_batch_tune(prt_ctx *ctx, const float *d_tris9, const float *d_rays6, void *d_out, int reps,
+                          double *before_ms, double *after_ms);
+
+/* The work tiles {first ray lo, hi, count, scene} the batch traversal claims, for given tile
+ * sizes per bucket, largest scenes first. Needs no device: *n_tiles receives the count; with
+ * tiles4 != NULL, *n_tiles must hold its capacity in tiles on entry. */
+int prt_mi355x_batch_tiles(uint32_t n_scenes, const uint64_t *tri_counts, const uint64_t *ray_counts,
+                           const int32_t chunk[4], uint32_t *tiles4, uint64_t *n_tiles);
+
 #ifdef __cplusplus
 }
 #endif

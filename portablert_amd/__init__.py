@@ -379,3 +379,96 @@ def mlp_top4(ctx, rays, with_logits=False):
     ctx._check(_learned_lib().prt_mi355x_mlp_top4(ctx.h, _p(rays), n, _p(top),
                                                    _p(logits) if with_logits else None))
     return (top, logits) if with_logits else top
+
+
+# ------------------------------------------------------------------ batched multi-scene tracing
+
+BATCH_BUCKETS = 4  # scenes by triangle count: < 4096, < 16384, < 65536, more
+
+
+def batch_bucket(tris):
+    return 0 if tris < 4096 else 1 if tris < 16384 else 2 if tris < 65536 else 3
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return L
+    vp, u32, ci = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.prt_mi355x_batch_setup.argtypes = [vp, u32, vp, vp, u32]
+    L.prt_mi355x_batch_run.argtypes = [vp, vp, vp, vp, vp]
+    L.prt_mi355x_batch_capture.argtypes = [vp, vp, vp, vp]
+    L.prt_mi355x_batch_replay.argtypes = [vp, vp]
+    L.prt_mi355x_batch_check.argtypes = [vp]
+    L.prt_mi355x_batch_get_params.argtypes = [vp, ci, vp]
+    L.prt_mi355x_batch_set_params.argtypes = [vp, ci, vp]
+    L.prt_mi355x_batch_tune.argtypes = [vp, vp, vp, vp, ci, dp, dp]
+    L.prt_mi355x_batch_tiles.argtypes = [u32, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    L._batch_bound = True
+    return L
+
+
+def _batch_lib():
+    return _bind_batch(lib())
+
+
+def batch_tiles(tri_counts, ray_counts, chunk=(64, 64, 64, 64)):
+    """Work tiles [n][4] = (first ray lo, hi, count, scene) of the batch traversal (no device)."""
+    tc = np.ascontiguousarray(tri_counts, np.uint64)
+    rc = np.ascontiguousarray(ray_counts, np.uint64)
+    ch = np.ascontiguousarray(chunk, np.int32)
+    n = ctypes.c_uint64(0)
+    L = _batch_lib()
+    rc_ = L.prt_mi355x_batch_tiles(len(tc), _p(tc), _p(rc), _p(ch), None, ctypes.byref(n))
+    if rc_:
+        raise PrtError(rc_, "batch_tiles: bad arguments")
+    out = np.zeros((n.value, 4), np.uint32)
+    rc_ = L.prt_mi355x_batch_tiles(len(tc), _p(tc), _p(rc), _p(ch), _p(out), ctypes.byref(n))
+    if rc_:
+        raise PrtError(rc_, "batch_tiles: bad arguments")
+    return out
+
+
+class Batch:
+    """Many scenes built and traced by one set of launches on a Context (device buffers)."""
+
+    def __init__(self, ctx, tri_counts, ray_counts, mask):
+        self.ctx = ctx
+        self.tri_counts = np.ascontiguousarray(tri_counts, np.uint64)
+        self.ray_counts = np.ascontiguousarray(ray_counts, np.uint64)
+        self.mask = mask
+        ctx._check(_batch_lib().prt_mi355x_batch_setup(ctx.h, len(self.tri_counts), _p(self.tri_counts),
+                                                       _p(self.ray_counts), mask))
+
+    def run(self, tris_ptr, rays_ptr, out_ptr, stream=None):
+        self.ctx._check(_batch_lib().prt_mi355x_batch_run(self.ctx.h, ctypes.c_void_p(tris_ptr),
+                                                          ctypes.c_void_p(rays_ptr), ctypes.c_void_p(out_ptr),
+                                                          ctypes.c_void_p(stream or 0)))
+
+    def capture(self, tris_ptr, rays_ptr, out_ptr):
+        self.ctx._check(_batch_lib().prt_mi355x_batch_capture(self.ctx.h, ctypes.c_void_p(tris_ptr),
+                                                              ctypes.c_void_p(rays_ptr), ctypes.c_void_p(out_ptr)))
+
+    def replay(self, stream=None):
+        self.ctx._check(_batch_lib().prt_mi355x_batch_replay(self.ctx.h, ctypes.c_void_p(stream or 0)))
+
+    def check(self):
+        self.ctx._check(_batch_lib().prt_mi355x_batch_check(self.ctx.h))
+
+    def params(self, bucket):
+        v = np.zeros(3, np.int32)
+        rc = _batch_lib().prt_mi355x_batch_get_params(self.ctx.h, bucket, _p(v))
+        if rc:
+            raise PrtError(rc, "batch_get_params")
+        return tuple(int(x) for x in v)
+
+    def set_params(self, bucket, values):
+        v = np.ascontiguousarray(values, np.int32)
+        self.ctx._check(_batch_lib().prt_mi355x_batch_set_params(self.ctx.h, bucket, _p(v)))
+
+    def tune(self, tris_ptr, rays_ptr, out_ptr, reps=3):
+        before, after = ctypes.c_double(), ctypes.c_double()
+        self.ctx._check(_batch_lib().prt_mi355x_batch_tune(self.ctx.h, ctypes.c_void_p(tris_ptr),
+                                                           ctypes.c_void_p(rays_ptr), ctypes.c_void_p(out_ptr), reps,
+                                                           ctypes.byref(before), ctypes.byref(after)))
+        return before.value, after.value

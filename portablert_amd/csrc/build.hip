@@ -3,15 +3,19 @@
 // for tie order).
 //
 // Pipeline, all on one stream:
-//   1. k_tri_prep     triangle boxes, centroid bounds, finiteness check
-//   2. k_morton       30-bit Morton code of the box centre (Karras 2012)
+//   1. k_tri_prep     triangle boxes, per-scene centroid bounds, finiteness check
+//   2. k_morton       30-bit Morton code of the box centre (Karras 2012) below the scene index
 //   3. radix sort     stable LSD, 8-bit digits, hand-written (hist / scan / scatter)
-//   4. k_karras       binary radix tree over the sorted codes (Karras 2012), index tiebreak
+//   4. k_karras       binary radix tree over the sorted codes (Karras 2012), index tiebreak;
+//                     the node spanning exactly one scene's keys is that scene's root
 //   5. k_bottomup     boxes and triangle counts leaf to root, second arrival continues
-//   6. k_collapse     per BVH8 level: open the child of largest surface area until 8 children,
-//                     subtrees of <= leaf_size triangles become leaves; writes fp32 and
+//   6. k_collapse     per BVH8 level: open the child of largest surface area until `width`
+//                     children, subtrees of <= leaf_size triangles become leaves; writes fp32 and
 //                     8-bit quantized nodes (bounds rounded outward and verified in fp32)
 //   7. k_emit_tris    leaf triangle records in the order the nodes address them
+//
+// Several scenes build together: the scene index sits above the Morton bits of each key, so
+// every scene is one subtree of the radix tree and BVH8 node s is the root of scene s.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,19 +31,20 @@ constexpr int kBlock = 256;
 constexpr int kSortItems = 16;                       // keys per thread per radix tile
 constexpr int kSortTile = kBlock * kSortItems;       // 4096 keys per tile
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kSceneShift = 30;                      // key = scene << 30 | 30-bit Morton code
 
 // misc buffer layout (uint32 words)
 enum : int {
-	M_CB = 0,          // 6 words: ordered centroid min xyz, max xyz
-	M_ERR = 6,         // non-finite input flag
-	M_NODES = 7,       // BVH8 node allocation counter
-	M_TRIS = 8,        // leaf triangle slot counter
-	M_LVL_BEGIN = 9,   // current level [begin, end)
-	M_LVL_END = 10,
-	M_DEPTH = 11,
-	M_SB = 12,         // 6 words: ordered scene box min xyz, max xyz
-	M_WORDS = 24
+	M_ERR = 0,         // bit 0 non-finite input, 1 triangle not placed, 2 levels exhausted
+	M_NODES = 1,       // BVH8 node allocation counter
+	M_TRIS = 2,        // leaf triangle slot counter
+	M_LVL_BEGIN = 3,   // current level [begin, end)
+	M_LVL_END = 4,
+	M_DEPTH = 5,
+	M_SCENE = 8,       // per scene 12 words: ordered centroid min xyz, max xyz, box min, max
+	M_PER_SCENE = 12
 };
+constexpr uint32_t misc_words(uint32_t scenes) { return M_SCENE + M_PER_SCENE * scenes; }
 
 __device__ __forceinline__ uint32_t f2o(float f) {
 	const uint32_t u = __float_as_uint(f);
@@ -49,23 +54,51 @@ __device__ __forceinline__ float o2f(uint32_t u) {
 	return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-__global__ void k_init_misc(uint32_t *misc) {
-	const int i = threadIdx.x;
-	if (i < M_WORDS) {
+__global__ __launch_bounds__(kBlock) void k_init_misc(uint32_t *misc, uint32_t words) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < words) {
 		uint32_t v = 0;
-		if (i < 3 || (i >= M_SB && i < M_SB + 3)) v = f2o(__builtin_inff());
-		else if (i < 6 || (i >= M_SB + 3 && i < M_SB + 6)) v = f2o(-__builtin_inff());
+		if (i >= M_SCENE) {
+			const uint32_t k = (i - M_SCENE) % M_PER_SCENE;  // min words start +inf, max words -inf
+			v = f2o(((k / 3) & 1u) ? -__builtin_inff() : __builtin_inff());
+		}
 		misc[i] = v;
 	}
 }
 
+// Scene of input triangle i: last s with off[s] <= i (off has scenes + 1 entries).
+__device__ __forceinline__ uint32_t scene_of(const uint32_t *off, uint32_t scenes, uint32_t i) {
+	uint32_t lo = 0, hi = scenes;  // off[lo] <= i < off[hi]
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (off[mid] <= i) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ void scene_atomics(uint32_t *w, const float c[3], const float cmax[3], const float smn[3],
+                                              const float smx[3]) {
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		atomicMin(w + k, f2o(c[k]));
+		atomicMax(w + 3 + k, f2o(cmax[k]));
+		atomicMin(w + 6 + k, f2o(smn[k]));
+		atomicMax(w + 9 + k, f2o(smx[k]));
+	}
+}
+
+// Triangle boxes (the scene index in the w word of the lower corner) and per-scene centroid and
+// vertex bounds: one set of atomics per wave when the wave lies in one scene.
 __global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ tris, uint32_t n,
+                                                     const uint32_t *__restrict__ off, uint32_t scenes,
                                                      float4 *__restrict__ tbox, uint32_t *misc) {
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	float c[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 	float cmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	float smn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 	float smx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	const uint32_t scene = (i < n && scenes > 1) ? scene_of(off, scenes, i) : 0u;
 	if (i < n) {
 		float v[9];
 #pragma unroll
@@ -83,30 +116,26 @@ __global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ t
 			smn[k] = lo[k];
 			smx[k] = hi[k];
 		}
-		tbox[2ull * i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+		tbox[2ull * i] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(scene));
 		tbox[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
 	}
-	// wave reduction, then one atomic per wave and axis
+	// lanes past n take the scene of lane 0 with empty bounds
+	const uint32_t s0 = __shfl(scene, 0);
+	if (__ballot(i < n && scene != s0) != 0ull) {
+		if (i < n) scene_atomics(misc + M_SCENE + M_PER_SCENE * scene, c, cmax, smn, smx);
+		return;
+	}
+	// wave reduction, then one atomic per wave and bound
 #pragma unroll
 	for (int k = 0; k < 3; ++k) {
-		for (int off = 32; off > 0; off >>= 1) {
-			c[k] = fminf(c[k], __shfl_xor(c[k], off));
-			cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off));
-			smn[k] = fminf(smn[k], __shfl_xor(smn[k], off));
-			smx[k] = fmaxf(smx[k], __shfl_xor(smx[k], off));
+		for (int sh = 32; sh > 0; sh >>= 1) {
+			c[k] = fminf(c[k], __shfl_xor(c[k], sh));
+			cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], sh));
+			smn[k] = fminf(smn[k], __shfl_xor(smn[k], sh));
+			smx[k] = fmaxf(smx[k], __shfl_xor(smx[k], sh));
 		}
 	}
-	if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-		for (int k = 0; k < 3; ++k) {
-			if (c[k] <= cmax[k]) {
-				atomicMin(misc + M_CB + k, f2o(c[k]));
-				atomicMax(misc + M_CB + 3 + k, f2o(cmax[k]));
-				atomicMin(misc + M_SB + k, f2o(smn[k]));
-				atomicMax(misc + M_SB + 3 + k, f2o(smx[k]));
-			}
-		}
-	}
+	if ((threadIdx.x & 63) == 0 && c[0] <= cmax[0]) scene_atomics(misc + M_SCENE + M_PER_SCENE * s0, c, cmax, smn, smx);
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {
@@ -124,17 +153,20 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4 *__restrict__ tb
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n) return;
 	const float4 lo = tbox[2ull * i], hi = tbox[2ull * i + 1];
+	const uint32_t scene = __float_as_uint(lo.w);
+	const uint32_t *cb = misc + M_SCENE + M_PER_SCENE * scene;
 	const float c[3] = {0.5f * lo.x + 0.5f * hi.x, 0.5f * lo.y + 0.5f * hi.y, 0.5f * lo.z + 0.5f * hi.z};
 	uint32_t q[3];
 #pragma unroll
 	for (int k = 0; k < 3; ++k) {
-		const float mn = o2f(misc[M_CB + k]), mx = o2f(misc[M_CB + 3 + k]);
+		const float mn = o2f(cb[k]), mx = o2f(cb[3 + k]);
 		const float ext = mx - mn;
 		float x = ext > 0.0f ? (c[k] - mn) / ext : 0.0f;
 		x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
 		q[k] = static_cast<uint32_t>(x);
 	}
-	keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+	keys[i] = (static_cast<uint64_t>(scene) << kSceneShift) | (spread10(q[0]) << 2) | (spread10(q[1]) << 1) |
+	          spread10(q[2]);
 	vals[i] = i;
 }
 
@@ -247,7 +279,8 @@ __device__ __forceinline__ int delta(const uint64_t *keys, uint32_t n, int i, in
 	return 64 + __clz(static_cast<uint32_t>(i) ^ static_cast<uint32_t>(j));
 }
 
-__global__ __launch_bounds__(kBlock) void k_karras(const uint64_t *__restrict__ keys, BinTree t) {
+__global__ __launch_bounds__(kBlock) void k_karras(const uint64_t *__restrict__ keys, BinTree t,
+                                                   uint32_t *__restrict__ scene_root) {
 	const int n = static_cast<int>(t.n);
 	const int i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n - 1) return;
@@ -273,9 +306,26 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint64_t *__restrict__ 
 	t.right[i] = rc;
 	t.parent[lc] = static_cast<uint32_t>(i);
 	t.parent[rc] = static_cast<uint32_t>(i);
-	t.first[i] = static_cast<uint32_t>(min(i, j));
+	const int lo = min(i, j), hi = max(i, j);
+	t.first[i] = static_cast<uint32_t>(lo);
 	t.flag[i] = 0;
 	if (i == 0) t.parent[0] = kNone;
+	// the keys of one scene share the bits above the Morton code and no other key does, so
+	// exactly one node spans them: the one whose range ends at both scene boundaries
+	const uint64_t sc = keys[lo] >> kSceneShift;
+	if ((keys[hi] >> kSceneShift) == sc && (lo == 0 || (keys[lo - 1] >> kSceneShift) != sc) &&
+	    (hi == n - 1 || (keys[hi + 1] >> kSceneShift) != sc))
+		scene_root[sc] = static_cast<uint32_t>(i);
+}
+
+// Scene roots before the radix tree: none for an empty scene, the leaf for a one-triangle
+// scene; scenes of two or more triangles get their internal root from k_karras.
+__global__ __launch_bounds__(kBlock) void k_scene_init(const uint32_t *__restrict__ off, uint32_t scenes, uint32_t n,
+                                                       uint32_t *__restrict__ scene_root) {
+	const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s >= scenes) return;
+	const uint32_t b = scenes > 1 ? off[s] : 0u, e = scenes > 1 ? off[s + 1] : n;
+	scene_root[s] = (e - b == 1u) ? (n - 1u + b) : kNone;
 }
 
 __device__ __forceinline__ float ld_coherent(const float *p) {
@@ -362,7 +412,10 @@ struct CollapseArgs {
 	uint32_t *tri_dest;
 	NodeQ *nq;
 	NodeF *nf;
+	const uint64_t *keys;   // sorted keys: the scene of binary node b is keys[first[b]] >> 30
+	const uint32_t *param;  // per scene leaf_size | width << 8, or null: leaf_size and width below
 	int leaf_size;
+	int width;
 };
 
 __global__ __launch_bounds__(kBlock) void k_level_advance(uint32_t *misc) {
@@ -380,11 +433,13 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 	const BinTree &t = a.t;
 	const uint32_t n_int = t.n - 1;  // binary internal nodes are ids < n_int
 	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
-		const uint32_t b = a.bin_of[i];
+		const uint32_t b = a.bin_of[i];  // kNone: root of an empty scene, written as an empty node
 		uint32_t el[8];
 		float ar[8];
 		int ne;
-		if (b < n_int) {
+		if (b == kNone) {
+			ne = 0;
+		} else if (b < n_int) {
 			el[0] = t.left[b];
 			el[1] = t.right[b];
 			ne = 2;
@@ -392,11 +447,19 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 			el[0] = b;
 			ne = 1;
 		}
+		int leaf_size = a.leaf_size, width = a.width;
+		if (a.param && b != kNone) {
+			const uint32_t pr = a.param[a.keys[t.first[b]] >> kSceneShift];
+			leaf_size = static_cast<int>(pr & 0xFFu);
+			width = static_cast<int>((pr >> 8) & 0xFFu);
+		}
+		leaf_size = max(1, min(kMaxLeaf, leaf_size));
+		width = max(2, min(8, width));
 #pragma unroll
 		for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
 		for (int k = 0; k < ne; ++k)
 			if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
-		while (ne < 8) {
+		while (ne < width) {
 			int best = -1;
 			float ba = -1.0f;
 #pragma unroll
@@ -420,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 		uint32_t n_leaf_tris = 0;
 		for (int k = 0; k < ne; ++k) {
 			cnt[k] = t.count[el[k]];
-			if (cnt[k] > static_cast<uint32_t>(a.leaf_size)) ++n_inner;
+			if (cnt[k] > static_cast<uint32_t>(leaf_size)) ++n_inner;
 			else n_leaf_tris += cnt[k];
 		}
 		const uint32_t child_base = n_inner ? atomicAdd(a.misc + M_NODES, static_cast<uint32_t>(n_inner)) : 0u;
@@ -439,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 		}
 		NodeQ q;
 		NodeF f;
+		if (ne == 0) nlo = nhi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		const float np[3] = {nlo.x, nlo.y, nlo.z}, nt[3] = {nhi.x, nhi.y, nhi.z};
 		float sc[3];
 #pragma unroll
@@ -461,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 				const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
 				lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
 				hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
-				if (cnt[k] > static_cast<uint32_t>(a.leaf_size)) {
+				if (cnt[k] > static_cast<uint32_t>(leaf_size)) {
 					meta = static_cast<uint8_t>(kMetaInternal | inner);
 					imask |= static_cast<uint8_t>(1u << k);
 					a.bin_of[child_base + inner] = el[k];
@@ -492,7 +556,9 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 	}
 }
 
+// Leaf triangle records; ids are scene-local (input index minus the scene's first index).
 __global__ __launch_bounds__(kBlock) void k_emit_tris(const float *__restrict__ tris, const uint32_t *__restrict__ sorted,
+                                                      const uint64_t *__restrict__ keys, const uint32_t *__restrict__ off,
                                                       const uint32_t *__restrict__ dest, uint32_t n,
                                                       TriRec *__restrict__ out, uint32_t *misc) {
 	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
@@ -505,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_tris(const float *__restrict__ 
 	TriRec r;
 #pragma unroll
 	for (int k = 0; k < 9; ++k) r.v[k] = tris[9ull * src + k];
-	r.id = src;
+	r.id = off ? src - off[keys[j] >> kSceneShift] : src;
 	r.pad[0] = r.pad[1] = 0;
 	out[dest[j]] = r;
 }
@@ -515,14 +581,31 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint32_t *p, uint32_t n, uint32
 	if (i < n) p[i] = v;
 }
 
-__global__ void k_root(uint32_t *misc, uint32_t *bin_of, uint32_t root) {
-	if (threadIdx.x == 0) {
-		bin_of[0] = root;
-		misc[M_NODES] = 1;
+// BVH8 level 0: node s is the root of scene s.
+__global__ __launch_bounds__(kBlock) void k_roots(uint32_t *misc, uint32_t *bin_of, const uint32_t *scene_root,
+                                                  uint32_t scenes) {
+	const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s < scenes) bin_of[s] = scene_root[s];
+	if (s == 0) {
+		misc[M_NODES] = scenes;
 		misc[M_LVL_BEGIN] = 0;
 		misc[M_LVL_END] = 0;
 		misc[M_DEPTH] = 0;
 	}
+}
+
+// After a fixed number of levels: flag a build the levels did not finish.
+__global__ void k_check_done(uint32_t *misc, uint32_t n) {
+	if (threadIdx.x == 0) {
+		if (misc[M_NODES] != misc[M_LVL_END]) atomicOr(misc + M_ERR, 4u);
+		if (misc[M_TRIS] != n) atomicOr(misc + M_ERR, 2u);
+	}
+}
+
+// Per-scene vertex bounds as floats: min xyz, max xyz.
+__global__ __launch_bounds__(kBlock) void k_scene_boxes(const uint32_t *misc, uint32_t scenes, float *box) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < 6 * scenes) box[i] = o2f(misc[M_SCENE + M_PER_SCENE * (i / 6) + 6 + i % 6]);
 }
 
 inline uint32_t blocks_for(uint64_t n, int per = kBlock) { return static_cast<uint32_t>((n + per - 1) / per); }
@@ -530,7 +613,7 @@ inline uint32_t blocks_for(uint64_t n, int per = kBlock) { return static_cast<ui
 }  // namespace
 
 hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 &out, hipStream_t st,
-                      bool *bad_input, double *build_ms) {
+                      bool *bad_input, double *build_ms, const SceneSet *scenes) {
 	auto h2f = [](uint32_t u) {
 		union {
 			uint32_t u;
@@ -544,15 +627,23 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		hipError_t e_ = (x);                                                                       \
 		if (e_ != hipSuccess) return e_;                                                           \
 	} while (0)
+	SceneSet one;
+	const SceneSet &ss = scenes ? *scenes : one;
+	const uint32_t S = ss.count;
+	const bool fixed = ss.levels > 0;  // no host synchronisation: capturable in a graph
 	*bad_input = false;
 	out.n_tris = n64;
 	out.n_nodes = 0;
 	out.depth = 0;
 	out.leaf_size = static_cast<uint32_t>(p.leaf_size);
+	if (S == 0 || S > kMaxScenes || (S > 1 && !ss.tri_off) || (fixed && build_ms)) return hipErrorInvalidValue;
 	if (n64 == 0) return hipSuccess;
 	if (n64 > static_cast<uint64_t>(kTriMask)) return hipErrorInvalidValue;
 	const uint32_t n = static_cast<uint32_t>(n64);
 	const uint32_t nb = 2 * n - 1;
+	const uint32_t cap = n + S;  // BVH8 nodes: at most one per binary internal node, one per scene root
+	const uint32_t words = misc_words(S);
+	const uint32_t *off = S > 1 ? ss.tri_off : nullptr;
 
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	if (build_ms) {
@@ -569,22 +660,26 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	PRT_TRY(s.hist.ensure(sizeof(uint32_t) * 256 * tiles));
 	PRT_TRY(s.bin.ensure(sizeof(uint32_t) * (6ull * nb + 8)));
 	PRT_TRY(s.bbox.ensure(sizeof(float4) * 2ull * nb));
-	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * n));
+	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * cap));
 	PRT_TRY(s.tri_dest.ensure(sizeof(uint32_t) * n));
-	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * M_WORDS));
-	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * M_WORDS));
-	PRT_TRY(out.nodes_q.ensure(sizeof(NodeQ) * n));
-	PRT_TRY(out.nodes_f.ensure(sizeof(NodeF) * n));
+	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * words));
+	PRT_TRY(s.scene_root.ensure(sizeof(uint32_t) * S));
+	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * misc_words(1)));
+	PRT_TRY(out.nodes_q.ensure(sizeof(NodeQ) * cap));
+	PRT_TRY(out.nodes_f.ensure(sizeof(NodeF) * cap));
 	PRT_TRY(out.tris.ensure(sizeof(TriRec) * n));
 
 	uint32_t *misc = s.misc.as<uint32_t>();
-	const float *tri_in = s.tri_in.as<float>();
-	hipLaunchKernelGGL(k_init_misc, dim3(1), dim3(64), 0, st, misc);
-	hipLaunchKernelGGL(k_tri_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, n, s.tbox.as<float4>(), misc);
+	const float *tri_in = ss.tris ? ss.tris : s.tri_in.as<float>();
+	hipLaunchKernelGGL(k_init_misc, dim3(blocks_for(words)), dim3(kBlock), 0, st, misc, words);
+	hipLaunchKernelGGL(k_tri_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, n, off, S, s.tbox.as<float4>(),
+	                   misc);
 	hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(), n, misc,
 	                   s.keys[0].as<uint64_t>(), s.vals[0].as<uint32_t>());
+	int key_bits = kSceneShift;
+	while (S > 1 && (1ull << (key_bits - kSceneShift)) < S) ++key_bits;
 	int cur = 0;
-	for (int shift = 0; shift < 30; shift += 8) {
+	for (int shift = 0; shift < key_bits; shift += 8) {
 		hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), n, shift,
 		                   s.hist.as<uint32_t>(), tiles);
 		hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.hist.as<uint32_t>(), 256 * tiles);
@@ -593,6 +688,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		                   n, shift, s.hist.as<uint32_t>(), tiles);
 		cur ^= 1;
 	}
+	const uint64_t *keys = s.keys[cur].as<uint64_t>();
 	BinTree t;
 	uint32_t *bw = s.bin.as<uint32_t>();
 	t.left = bw;
@@ -603,13 +699,14 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	t.flag = bw + 5ull * nb;
 	t.box = s.bbox.as<float4>();
 	t.n = n;
+	uint32_t *scene_root = s.scene_root.as<uint32_t>();
+	hipLaunchKernelGGL(k_scene_init, dim3(blocks_for(S)), dim3(kBlock), 0, st, off, S, n, scene_root);
 	if (n > 1)
-		hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), t);
+		hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, keys, t, scene_root);
 	hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
 	                   s.vals[cur].as<uint32_t>(), t);
 	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
-	// BVH8 levels; the root is binary node 0 (or leaf 0 when n == 1)
-	hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, misc, s.bin_of.as<uint32_t>(), 0u);
+	hipLaunchKernelGGL(k_roots, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, s.bin_of.as<uint32_t>(), scene_root, S);
 	CollapseArgs ca;
 	ca.t = t;
 	ca.misc = misc;
@@ -617,23 +714,35 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	ca.tri_dest = s.tri_dest.as<uint32_t>();
 	ca.nq = out.nodes_q.as<NodeQ>();
 	ca.nf = out.nodes_f.as<NodeF>();
+	ca.keys = keys;
+	ca.param = ss.param;
 	ca.leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size));
+	ca.width = std::max(2, std::min(8, p.width));
 	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
-	for (int level = 0; level < 128; ++level) {
+	const int max_levels = fixed ? ss.levels : 128;
+	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
+	for (int level = 0; level < max_levels; ++level) {
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
-		const uint32_t grid = std::min<uint32_t>(blocks_for(n), 2048);
 		hipLaunchKernelGGL(k_collapse, dim3(grid), dim3(kBlock), 0, st, ca);
-		if ((level & 3) == 3) {
-			PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * M_WORDS, hipMemcpyDeviceToHost, st));
+		if (!fixed && (level & 3) == 3) {
+			PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
 			PRT_TRY(hipStreamSynchronize(st));
 			// done when the level just processed created no new node
 			if (hm[M_NODES] == hm[M_LVL_END]) break;
 		}
 	}
-	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(),
-	                   s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>(), misc);
+	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(), keys,
+	                   off, s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>(), misc);
+	if (ss.box) hipLaunchKernelGGL(k_scene_boxes, dim3(blocks_for(6ull * S)), dim3(kBlock), 0, st, misc, S, ss.box);
+	if (fixed) {
+		hipLaunchKernelGGL(k_check_done, dim3(1), dim3(64), 0, st, misc, n);
+		PRT_TRY(hipGetLastError());
+		out.n_nodes = cap;  // an upper bound: the device holds the count
+		out.depth = static_cast<uint32_t>(ss.levels);
+		return hipSuccess;
+	}
 	PRT_TRY(hipGetLastError());
-	PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * M_WORDS, hipMemcpyDeviceToHost, st));
+	PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
 	if (build_ms) PRT_TRY(hipEventRecord(ev1, st));
 	PRT_TRY(hipStreamSynchronize(st));
 	if (build_ms) {
@@ -645,12 +754,22 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	}
 	*bad_input = (hm[M_ERR] & 1u) != 0;
 	if (hm[M_ERR] & 2u) return hipErrorUnknown;  // inconsistent build
-	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SB + k]);
+	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SCENE + 6 + k]);
 	out.n_nodes = hm[M_NODES];
 	out.depth = hm[M_DEPTH];
 	if (hm[M_TRIS] != n || hm[M_NODES] != hm[M_LVL_END]) return hipErrorUnknown;  // incomplete build
 	return hipSuccess;
 #undef PRT_TRY
+}
+
+hipError_t build_errors(BuildScratch &s, hipStream_t st, uint32_t *flags) {
+	*flags = 0;
+	if (!s.misc.ptr || !s.host_misc) return hipSuccess;
+	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
+	hipError_t e = hipMemcpyAsync(hm, s.misc.ptr, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st);
+	if (e == hipSuccess) e = hipStreamSynchronize(st);
+	if (e == hipSuccess) *flags = hm[M_ERR];
+	return e;
 }
 
 }  // namespace prt

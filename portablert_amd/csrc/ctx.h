@@ -1,11 +1,13 @@
-// The C ABI context (prt_ctx) shared by api.hip and learned.hip.
+// The C ABI context (prt_ctx) shared by api.hip, learned.hip and batch.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 
 #include "../../include/prt_mi355x.h"
+#include "batch.h"
 #include "internal.h"
 #include "mlp.h"
 
@@ -34,6 +36,8 @@ struct prt_ctx {
 	double seed_top1 = 0.0, seed_top4 = 0.0;
 	uint64_t learned_scene = 0;  // set_tris generation the clusters were built for
 	uint64_t scene_gen = 0;
+	// batched multi-scene tracing
+	std::unique_ptr<prt::Batch> batch;
 };
 
 namespace prt {

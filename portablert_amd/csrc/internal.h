@@ -61,18 +61,42 @@ struct BuildScratch {
 	DevBuf bbox;        // float4[2*(2n-1)] binary node boxes
 	DevBuf bin_of;      // BVH8 node -> binary node
 	DevBuf tri_dest;    // sorted position -> TriRec slot
-	DevBuf misc;        // counters and scene bounds
-	void *host_misc = nullptr;  // pinned mirror of misc
+	DevBuf misc;        // counters and per-scene bounds
+	DevBuf scene_root;  // binary root node of every scene
+	void *host_misc = nullptr;  // pinned mirror of the counters and scene 0's bounds
 };
 
 struct BuildParams {
 	int leaf_size = 2;  // 1..kMaxLeaf
+	int width = 8;      // children per BVH8 node, 2..8 (the node format has 8 slots)
 };
 
-// Builds `out` from n triangles already in scratch.tri_in (device, 9 floats each).
-// Returns hipSuccess, or an error; *bad_input is set when a vertex is not finite.
+constexpr uint32_t kMaxScenes = 65536;
+
+// Several scenes in one build (device arrays). BVH8 node s is the root of scene s (an empty node
+// for an empty scene) and leaf triangle records carry scene-local ids.
+struct SceneSet {
+	uint32_t count = 1;
+	const uint32_t *tri_off = nullptr;  // count + 1 prefix offsets into the triangle array
+	const uint32_t *param = nullptr;    // per scene leaf_size | width << 8 (null: BuildParams)
+	float *box = nullptr;               // out: per scene vertex bounds, min xyz max xyz, or null
+	const float *tris = nullptr;        // triangles (null: the scratch's tri_in)
+	int levels = 0;                     // > 0: exactly this many BVH8 levels and no host
+	                                    // synchronisation, so the build can be captured in a
+	                                    // graph once its buffers exist; completion is checked
+	                                    // on the device
+};
+
+// Builds `out` from n triangles (device, 9 floats each; scratch.tri_in unless the scene set
+// names them). Returns hipSuccess, or an error; *bad_input is set when a vertex is not finite.
+// With scenes->levels > 0, out.n_nodes is the node capacity, out.depth the level count, and
+// errors are read afterwards with build_errors.
 hipError_t build_bvh8(BuildScratch &s, uint64_t n, const BuildParams &p, Bvh8 &out, hipStream_t st,
-                      bool *bad_input, double *build_ms);
+                      bool *bad_input, double *build_ms, const SceneSet *scenes = nullptr);
+
+// Error bits of the last build on `s`: 1 non-finite vertex, 2 triangle not placed, 4 BVH8 levels
+// exhausted before the build finished. Synchronises st.
+hipError_t build_errors(BuildScratch &s, hipStream_t st, uint32_t *flags);
 
 // Traversal tunables.
 struct TraceParams {
@@ -116,8 +140,19 @@ struct SeedArgs {
 	uint32_t count;
 };
 
+// Several scenes in one launch over a BVH built from a SceneSet: rays are claimed in tiles
+// {first lo, first hi, count, scene} (each inside one scene), and info holds per scene
+// {absolute culling margin, 1 if every vertex is below 2^60 in magnitude else 0}. n is the total
+// ray count and records go to out + ray index * stride.
+struct BatchArgs {
+	const uint4 *tiles;
+	uint32_t n_tiles;
+	const float2 *info;
+	uint32_t n_scenes;
+};
+
 hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
                       uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats,
-                      const SeedArgs *seeds = nullptr);
+                      const SeedArgs *seeds = nullptr, const BatchArgs *batch = nullptr);
 
 }  // namespace prt

@@ -49,7 +49,14 @@ struct KArgs {
 	const uint32_t *cl_entry;
 	const float *cl_box;
 	uint32_t n_clusters;
+	// several scenes (null when off): ray tiles {first lo, first hi, count, scene} claimed one
+	// per atomic, and per scene {margin_abs, fast_scene}; BVH8 node s is the root of scene s
+	const uint4 *tiles;
+	uint32_t n_tiles;
+	const float2 *scene_info;
 };
+
+enum : int { kPlain = 0, kSeeded = 1, kBatch = 2 };
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
@@ -334,8 +341,9 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 	}
 }
 
-template <class Node, int S, int B, bool COUNT, bool SEEDED>
+template <class Node, int S, int B, bool COUNT, int MODE>
 __global__ __launch_bounds__(B) void k_trace(KArgs a) {
+	constexpr bool SEEDED = MODE == kSeeded, BATCH = MODE == kBatch;
 	__shared__ uint2 lds[S * B];
 	const Node *nodes = static_cast<const Node *>(a.nodes);
 	const uint32_t lane = threadIdx.x & 63u;
@@ -357,6 +365,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 	bool root_phase = true;
 	// wave-uniform ray chunk [chunk_next, chunk_end) reserved with one atomic
 	uint64_t chunk_next = 0, chunk_end = 0;
+	uint32_t chunk_scene = 0;
 	bool exhausted = false;
 	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0, c_nphase = 0, c_tphase = 0, c_culled = 0;
 
@@ -364,7 +373,20 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle)
 		const uint64_t need = __ballot(!active);
 		if (need != 0ull && !exhausted) {
-			if (chunk_next >= chunk_end) {
+			if (chunk_next >= chunk_end && BATCH) {
+				uint64_t tix = 0;
+				if (lane == 0) tix = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter), 1ull);
+				tix = __shfl(tix, 0);
+				if (tix < a.n_tiles) {
+					const uint4 tl = a.tiles[tix];
+					chunk_next = static_cast<uint64_t>(tl.x) | (static_cast<uint64_t>(tl.y) << 32);
+					chunk_end = chunk_next + tl.z;
+					chunk_scene = tl.w;
+				} else {
+					exhausted = true;
+					chunk_next = chunk_end = 0;
+				}
+			} else if (chunk_next >= chunk_end) {
 				uint64_t base = 0;
 				if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter),
 				                                static_cast<unsigned long long>(a.chunk));
@@ -377,9 +399,15 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 			const uint32_t rank = __popcll(need & lt_mask);
 			if (!active && rank < avail) {
 				ray_idx = chunk_next + rank;
-				load_ray(a.rays, ray_idx, a.margin_abs, a.fast_scene != 0, r);
 				active = true;
-				cur = a.n_nodes > 0 ? 0u : kInvalid;
+				if (BATCH) {
+					const float2 si = a.scene_info[chunk_scene];
+					load_ray(a.rays, ray_idx, si.x, si.y != 0.0f, r);
+					cur = chunk_scene;
+				} else {
+					load_ray(a.rays, ray_idx, a.margin_abs, a.fast_scene != 0, r);
+					cur = a.n_nodes > 0 ? 0u : kInvalid;
+				}
 				pend = kInvalid;
 				st.sp = 0;
 				if (SEEDED && a.n_nodes > 0) {
@@ -520,17 +548,19 @@ __global__ void k_zero(unsigned long long *p, int n) {
 	if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }
 
+template <class Node, int S, int B, int MODE>
+void launch_m(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
+	if (count)
+		hipLaunchKernelGGL((k_trace<Node, S, B, true, MODE>), dim3(grid), dim3(B), 0, st, a);
+	else
+		hipLaunchKernelGGL((k_trace<Node, S, B, false, MODE>), dim3(grid), dim3(B), 0, st, a);
+}
+
 template <class Node, int S, int B>
 hipError_t launch(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
-	const bool seeded = a.seeds != nullptr;
-	if (count && seeded)
-		hipLaunchKernelGGL((k_trace<Node, S, B, true, true>), dim3(grid), dim3(B), 0, st, a);
-	else if (count)
-		hipLaunchKernelGGL((k_trace<Node, S, B, true, false>), dim3(grid), dim3(B), 0, st, a);
-	else if (seeded)
-		hipLaunchKernelGGL((k_trace<Node, S, B, false, true>), dim3(grid), dim3(B), 0, st, a);
-	else
-		hipLaunchKernelGGL((k_trace<Node, S, B, false, false>), dim3(grid), dim3(B), 0, st, a);
+	if (a.tiles) launch_m<Node, S, B, kBatch>(a, grid, count, st);
+	else if (a.seeds) launch_m<Node, S, B, kSeeded>(a, grid, count, st);
+	else launch_m<Node, S, B, kPlain>(a, grid, count, st);
 	return hipGetLastError();
 }
 
@@ -556,7 +586,8 @@ hipError_t launch_b(const KArgs &a, uint32_t grid, int B, int S, bool count, hip
 }  // namespace
 
 hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
-                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats, const SeedArgs *seeds) {
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats, const SeedArgs *seeds,
+                      const BatchArgs *batch) {
 #define PRT_TRY(x)                                                                                 \
 	do {                                                                                           \
 		hipError_t e_ = (x);                                                                       \
@@ -603,6 +634,10 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.cl_entry = seeds ? seeds->entry : nullptr;
 	a.cl_box = seeds ? seeds->box : nullptr;
 	a.n_clusters = seeds ? seeds->count : 0u;
+	a.tiles = batch ? batch->tiles : nullptr;
+	a.n_tiles = batch ? batch->n_tiles : 0u;
+	a.scene_info = batch ? batch->info : nullptr;
+	if (batch && (seeds || b.n_nodes < batch->n_scenes || !batch->tiles || !batch->info)) return hipErrorInvalidValue;
 	a.fast_scene = 1;
 	for (int k = 0; k < 6; ++k) a.fast_scene &= std::fabs(b.scene_box[k]) < 0x1p60f ? 1 : 0;
 

@@ -141,3 +141,31 @@ def test_reference_examples_compile_unchanged(name, tmp_path):
                     "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
+
+
+def test_batch_tiles_cover_every_ray_once():
+    """The work tiles of the batch traversal: each inside one scene, every ray exactly once,
+    tile sizes per size bucket, scenes with more triangles first (ties by index)."""
+    rng = np.random.default_rng(5)
+    tri = rng.integers(0, 300000, 40)
+    tri[3], tri[4] = 0, 1
+    ray = rng.integers(0, 5000, 40)
+    ray[7] = 0
+    chunk = (7, 64, 1, 500)
+    t = prt.batch_tiles(tri, ray, chunk)
+    ray_off = np.concatenate([[0], np.cumsum(ray)])
+    first = t[:, 0].astype(np.int64) | (t[:, 1].astype(np.int64) << 32)
+    cnt, scene = t[:, 2].astype(np.int64), t[:, 3]
+    assert (cnt >= 1).all()
+    assert (first >= ray_off[scene]).all() and (first + cnt <= ray_off[scene + 1]).all()
+    cover = np.zeros(ray_off[-1], np.int32)
+    for f, c in zip(first, cnt):
+        cover[f:f + c] += 1
+    assert (cover == 1).all()
+    for s in range(40):
+        c = cnt[scene == s]
+        want = chunk[prt.batch_bucket(int(tri[s]))]
+        assert (c[:-1] == want).all() and (len(c) == 0 or c[-1] <= want)
+    order = [int(s) for k, s in enumerate(scene) if k == 0 or scene[k - 1] != s]
+    assert order == sorted(order, key=lambda s: (-int(tri[s]), s))
+    assert len(prt.batch_tiles([], [], chunk)) == 0
