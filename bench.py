@@ -130,6 +130,131 @@ def learned_leg(ctx, tris, steps, warmup, epochs):
     return res
 
 
+def _timed(fn, steps, warmup, stream):
+    """Mean ms per call of fn over `steps` calls, bracketed by events on `stream`."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def batch_leg(ctx, steps, warmup, stream):
+    """64 scenes of 1,011..191,723 triangles with 10,948..1,979,349 rays each (2,516,661
+    triangles, 24,428,377 rays): every BVH build and the one persistent traversal launch replayed
+    as one hipGraph, after the autotuner. Compared with the same work scene by scene through
+    set_tris + nearest_hits (a host synchronisation per build)."""
+    import torch
+
+    import portablert_amd as prt
+    from portablert_amd import scenes
+    n_sc = scenes.n_scenes(3)
+    tl = [scenes.scene(3, s) for s in range(n_sc)]
+    rl = [scenes.rays(3, s) for s in range(n_sc)]
+    tc, rc = [len(t) for t in tl], [len(r) for r in rl]
+    d_tris = torch.from_numpy(np.concatenate(tl)).cuda()
+    d_rays = torch.from_numpy(np.concatenate(rl)).cuda()
+    del tl, rl
+    n = int(sum(rc))
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    d_out = torch.zeros(n * size, dtype=torch.uint8, device="cuda")
+    d_seq = torch.zeros_like(d_out)
+    torch.cuda.synchronize()
+    sp = stream.cuda_stream
+    batch = prt.Batch(ctx, tc, rc, mask)
+    default_params = [batch.params(k) for k in range(-1, prt.BATCH_BUCKETS)]
+    batch.run(d_tris.data_ptr(), d_rays.data_ptr(), d_out.data_ptr(), sp)
+    batch.check()
+    t0 = time.perf_counter()
+    before, after = batch.tune(d_tris.data_ptr(), d_rays.data_ptr(), d_out.data_ptr(), reps=2)
+    tune_s = time.perf_counter() - t0
+    params = [batch.params(k) for k in range(-1, prt.BATCH_BUCKETS)]
+    eager_ms = _timed(lambda: batch.run(d_tris.data_ptr(), d_rays.data_ptr(), d_out.data_ptr(), sp), steps, warmup,
+                      stream)
+    batch.check()
+    batch.capture(d_tris.data_ptr(), d_rays.data_ptr(), d_out.data_ptr())
+    graph_ms = _timed(lambda: batch.replay(sp), steps, warmup, stream)
+    batch.check()
+    # the same work one scene at a time through the single-scene entry points
+    tri_off = np.concatenate([[0], np.cumsum(tc)])
+    ray_off = np.concatenate([[0], np.cumsum(rc)])
+
+    def sequential():
+        for s in range(n_sc):
+            ctx.set_tris_device(d_tris[tri_off[s]:].data_ptr(), tc[s])
+            ctx.nearest_hits_device(d_rays[ray_off[s]:].data_ptr(), rc[s], mask, d_seq[ray_off[s] * size:].data_ptr(), sp)
+        torch.cuda.synchronize()
+
+    sequential()
+    t0 = time.perf_counter()
+    reps = 2
+    for _ in range(reps):
+        sequential()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / reps
+    identical = bool(torch.equal(d_out, d_seq))
+    log(f"batch: graph {graph_ms:.2f} ms, eager {eager_ms:.2f} ms, scene by scene {seq_ms:.2f} ms; "
+        f"tuner {before:.2f} -> {after:.2f} ms in {tune_s:.1f} s; identical {identical}")
+    res = {
+        "workload": f"{n_sc} scenes, {int(sum(tc))} triangles, {n} rays (uniform-sphere rays from interior points), "
+                    "HitReg<t, primitive_id>; builds + traversal per step",
+        "graph_ms": round(graph_ms, 3),
+        "graph_mrays_s": round(n / graph_ms / 1e3, 2),
+        "eager_ms": round(eager_ms, 3),
+        "scene_by_scene_ms": round(seq_ms, 3),
+        "speedup_vs_scene_by_scene": round(seq_ms / graph_ms, 2),
+        "identical_to_scene_by_scene": identical,
+        "tuner_before_ms": round(before, 3),
+        "tuner_after_ms": round(after, 3),
+        "tuner_s": round(tune_s, 2),
+        "params_default": {"launch": default_params[0], "buckets": default_params[1:]},
+        "params_tuned": {"launch [block, waves/CU, LDS stack]": params[0],
+                         "buckets [width, leaf, tile]": params[1:]},
+    }
+    del d_tris, d_rays, d_out, d_seq, batch
+    torch.cuda.empty_cache()
+    return res
+
+
+def icosphere_leg(ctx, steps, warmup, stream):
+    """Closed displaced icosphere (81,920 triangles): 1920x1080 primary rays, then 4
+    cosine-weighted bounce rays per hit, every HitReg field (32-byte records)."""
+    import torch
+
+    import portablert_amd as prt
+    from portablert_amd import scenes
+    tris = scenes.scene(1)
+    prim = scenes.rays(1)
+    mask = prt.mask_of(prt.ALL_TAGS)
+    ctx.set_tris(tris)
+    hits = ctx.nearest_hits(prim, mask)
+    bounce = scenes.bounce_rays(tris, prim, hits["t"], hits["primitive_id"],
+                                np.stack([hits["px"], hits["py"], hits["pz"]], 1))
+    size, _ = prt.layout(mask)
+    sp = stream.cuda_stream
+    res = {"workload": f"icosphere {len(tris)} triangles, {len(prim)} primary + {len(bounce)} bounce rays, "
+                       "all HitReg fields"}
+    for name, r in (("primary", prim), ("bounce", bounce)):
+        d_r = torch.from_numpy(r).cuda()
+        d_o = torch.zeros(len(r) * size, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        ms = _timed(lambda: ctx.nearest_hits_device(d_r.data_ptr(), len(r), mask, d_o.data_ptr(), sp), steps, warmup,
+                    stream)
+        res[f"{name}_rays"] = len(r)
+        res[f"{name}_ms"] = round(ms, 3)
+        res[f"{name}_mrays_s"] = round(len(r) / ms / 1e3, 2)
+    total = len(prim) + len(bounce)
+    res["total_mrays_s"] = round(total / (res["primary_ms"] + res["bounce_ms"]) / 1e3, 2)
+    log(f"icosphere: primary {res['primary_mrays_s']} Mrays/s, bounce {res['bounce_mrays_s']} Mrays/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +267,8 @@ def main():
                     help="also run a 512 MiB elementwise copy kernel (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding leg")
     ap.add_argument("--epochs", type=int, default=2, help="training epochs of the learned leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
+    ap.add_argument("--no-icosphere", action="store_true", help="skip the icosphere primary + bounce leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU)",
@@ -261,6 +388,10 @@ def main():
         lr = learned_leg(ctx, tris, max(3, args.steps // 4), 1, args.epochs)
         out["learned_mrays_s"] = lr["learned_mrays_s"]
         out["learned"] = lr
+    if not args.no_batch:
+        out["batch"] = batch_leg(ctx, max(3, args.steps // 4), 1, stream)
+    if not args.no_icosphere:
+        out["icosphere"] = icosphere_leg(ctx, max(3, args.steps // 4), 1, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tris, rays)
     print(json.dumps(out), flush=True)

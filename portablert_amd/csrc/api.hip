@@ -119,6 +119,11 @@ int prt_mi355x_create(int device, prt_ctx **out) {
 		delete c;
 		return PRT_ERR_HIP;
 	}
+	if (hipEventCreateWithFlags(&c->busy, hipEventDisableTiming) != hipSuccess) {
+		(void)hipStreamDestroy(c->stream);
+		delete c;
+		return PRT_ERR_HIP;
+	}
 	*out = c;
 	return PRT_OK;
 }
@@ -137,6 +142,8 @@ void prt_mi355x_destroy(prt_ctx *c) {
 	for (auto *b : bufs) b->release();
 	if (c->bs.host_misc) (void)hipHostFree(c->bs.host_misc);
 	if (c->ts.host_err) (void)hipHostFree(c->ts.host_err);
+	c->batch.reset();
+	if (c->busy) (void)hipEventDestroy(c->busy);
 	(void)hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -155,7 +162,8 @@ int prt_mi355x_set_tris(prt_ctx *c, const float *tris, uint64_t n) {
 	if (!tris && n) return fail(c, PRT_ERR_ARG, "set_tris: null triangle pointer");
 	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
 	if (int r = over_limit(c, n * 36ull, "set_tris")) return r;
-	hipError_t e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
+	hipError_t e = prt::wait_use(c);  // traversals on caller streams still read the old BVH
+	if (e == hipSuccess) e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
 	if (e != hipSuccess) return hip_fail(c, e, "set_tris: triangle buffer");
 	if (n) {
 		e = hipMemcpyAsync(c->bs.tri_in.ptr, tris, n * 36ull, hipMemcpyHostToDevice, c->stream);
@@ -170,7 +178,8 @@ int prt_mi355x_set_tris_device(prt_ctx *c, const float *d_tris, uint64_t n) {
 	if (!d_tris && n) return fail(c, PRT_ERR_ARG, "set_tris_device: null triangle pointer");
 	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
 	if (int r = over_limit(c, n * 36ull, "set_tris")) return r;
-	hipError_t e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
+	hipError_t e = prt::wait_use(c);  // traversals on caller streams still read the old BVH
+	if (e == hipSuccess) e = c->bs.tri_in.ensure(std::max<uint64_t>(1, n) * 36ull);
 	if (e != hipSuccess) return hip_fail(c, e, "set_tris: triangle buffer");
 	if (n) {
 		e = hipMemcpyAsync(c->bs.tri_in.ptr, d_tris, n * 36ull, hipMemcpyDeviceToDevice, c->stream);
@@ -184,6 +193,7 @@ static int trace_common(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t ma
 	prt::TraceStats stats;
 	hipError_t e = prt::trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st,
 	                               c->time_kernel ? &ms : nullptr, c->tp.count_stats ? &stats : nullptr);
+	if (e == hipSuccess) e = prt::mark_use(c, st);
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: traversal");
 	if (c->time_kernel) c->trace_ms = ms;
 	if (c->tp.count_stats) c->stats = stats;

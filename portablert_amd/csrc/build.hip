@@ -8,7 +8,8 @@
 //   3. radix sort     stable LSD, 8-bit digits, hand-written (hist / scan / scatter)
 //   4. k_karras       binary radix tree over the sorted codes (Karras 2012), index tiebreak;
 //                     the node spanning exactly one scene's keys is that scene's root
-//   5. k_bottomup     boxes and triangle counts leaf to root, second arrival continues
+//   5. k_bottomup     boxes leaf to root, second arrival continues (counts come from the
+//                     Karras ranges)
 //   6. k_collapse     per BVH8 level: open the child of largest surface area until `width`
 //                     children, subtrees of <= leaf_size triangles become leaves; writes fp32 and
 //                     8-bit quantized nodes (bounds rounded outward and verified in fp32)
@@ -308,6 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint64_t *__restrict__ 
 	t.parent[rc] = static_cast<uint32_t>(i);
 	const int lo = min(i, j), hi = max(i, j);
 	t.first[i] = static_cast<uint32_t>(lo);
+	t.count[i] = static_cast<uint32_t>(hi - lo + 1);
 	t.flag[i] = 0;
 	if (i == 0) t.parent[0] = kNone;
 	// the keys of one scene share the bits above the Morton code and no other key does, so
@@ -328,13 +330,32 @@ __global__ __launch_bounds__(kBlock) void k_scene_init(const uint32_t *__restric
 	scene_root[s] = (e - b == 1u) ? (n - 1u + b) : kNone;
 }
 
-__device__ __forceinline__ float ld_coherent(const float *p) {
-	return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED,
-	                                         __HIP_MEMORY_SCOPE_AGENT));
+// Boxes that one thread writes and a thread of another workgroup reads in the same launch go
+// through device-scope atomic 8-byte stores and loads, which are coherent across the XCDs'
+// separate L2 caches without writing back a whole L2 as a device-scope fence does.
+__device__ __forceinline__ uint64_t pack2(float a, float b) {
+	return static_cast<uint64_t>(__float_as_uint(a)) | (static_cast<uint64_t>(__float_as_uint(b)) << 32);
 }
-__device__ __forceinline__ uint32_t ld_coherent(const uint32_t *p) {
-	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_box(float4 *box, uint32_t node, float4 lo, float4 hi) {
+	uint64_t *p = reinterpret_cast<uint64_t *>(box + 2ull * node);
+	__hip_atomic_store(p + 0, pack2(lo.x, lo.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(p + 1, pack2(lo.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(p + 2, pack2(hi.x, hi.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(p + 3, pack2(hi.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void ld_box(const float4 *box, uint32_t node, float4 &lo, float4 &hi) {
+	const uint64_t *p = reinterpret_cast<const uint64_t *>(box + 2ull * node);
+	uint64_t w[4];
+#pragma unroll
+	for (int k = 0; k < 4; ++k) w[k] = __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	lo = make_float4(__uint_as_float(static_cast<uint32_t>(w[0])), __uint_as_float(static_cast<uint32_t>(w[0] >> 32)),
+	                 __uint_as_float(static_cast<uint32_t>(w[1])), 0.0f);
+	hi = make_float4(__uint_as_float(static_cast<uint32_t>(w[2])), __uint_as_float(static_cast<uint32_t>(w[2] >> 32)),
+	                 __uint_as_float(static_cast<uint32_t>(w[3])), 0.0f);
+}
+// Waits until this lane's earlier stores are complete (device-scope atomic stores are then
+// visible to every CU), so the flag increment that follows publishes them.
+__device__ __forceinline__ void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ tbox,
                                                      const uint32_t *__restrict__ sorted, BinTree t) {
@@ -342,32 +363,27 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	if (j >= t.n) return;
 	uint32_t node = t.n - 1 + j;
 	const uint32_t tri = sorted[j];
-	t.box[2ull * node] = tbox[2ull * tri];
-	t.box[2ull * node + 1] = tbox[2ull * tri + 1];
+	float4 lo = tbox[2ull * tri], hi = tbox[2ull * tri + 1];
+	lo.w = hi.w = 0.0f;
+	st_box(t.box, node, lo, hi);
 	t.count[node] = 1;
 	t.first[node] = j;
-	__threadfence();
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
-	// a radix tree over at most 2^29 keys is at most 64 + 29 levels deep
+	// a radix tree over at most 2^29 keys of at most 46 bits is at most 46 + 29 levels deep
 	for (int guard = 0; p != kNone && guard < 128; ++guard) {
+		stores_done();
 		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
-		__threadfence();
-		const uint32_t a = t.left[p], b = t.right[p];
-		const float *ba = reinterpret_cast<const float *>(t.box + 2ull * a);
-		const float *bb = reinterpret_cast<const float *>(t.box + 2ull * b);
-		float4 lo, hi;
-		lo.x = fminf(ld_coherent(ba + 0), ld_coherent(bb + 0));
-		lo.y = fminf(ld_coherent(ba + 1), ld_coherent(bb + 1));
-		lo.z = fminf(ld_coherent(ba + 2), ld_coherent(bb + 2));
-		lo.w = 0.0f;
-		hi.x = fmaxf(ld_coherent(ba + 4), ld_coherent(bb + 4));
-		hi.y = fmaxf(ld_coherent(ba + 5), ld_coherent(bb + 5));
-		hi.z = fmaxf(ld_coherent(ba + 6), ld_coherent(bb + 6));
-		hi.w = 0.0f;
-		t.box[2ull * p] = lo;
-		t.box[2ull * p + 1] = hi;
-		t.count[p] = ld_coherent(t.count + a) + ld_coherent(t.count + b);
-		__threadfence();
+		const uint32_t sib = (t.left[p] == node) ? t.right[p] : t.left[p];
+		float4 sl, sh;
+		ld_box(t.box, sib, sl, sh);
+		lo.x = fminf(lo.x, sl.x);
+		lo.y = fminf(lo.y, sl.y);
+		lo.z = fminf(lo.z, sl.z);
+		hi.x = fmaxf(hi.x, sh.x);
+		hi.y = fmaxf(hi.y, sh.y);
+		hi.z = fmaxf(hi.z, sh.z);
+		st_box(t.box, p, lo, hi);
+		node = p;
 		p = t.parent[p];
 	}
 }

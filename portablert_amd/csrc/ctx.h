@@ -14,6 +14,9 @@
 struct prt_ctx {
 	int device = 0;
 	hipStream_t stream = nullptr;
+	// recorded after a traversal enqueued on a caller stream; rebuilding the BVH waits for it
+	hipEvent_t busy = nullptr;
+	bool busy_set = false;
 	std::string name;
 	std::string err;
 	prt::Bvh8 bvh;
@@ -45,6 +48,20 @@ namespace prt {
 inline int fail(prt_ctx *c, int code, const std::string &msg) {
 	if (c) c->err = msg;
 	return code;
+}
+
+// Marks the BVH as read by work enqueued on `st` (when st is not the context's stream).
+inline hipError_t mark_use(prt_ctx *c, hipStream_t st) {
+	if (st == c->stream || !c->busy) return hipSuccess;
+	c->busy_set = true;
+	return hipEventRecord(c->busy, st);
+}
+
+// Makes the context's stream wait for the last traversal enqueued on a caller stream.
+inline hipError_t wait_use(prt_ctx *c) {
+	if (!c->busy_set) return hipSuccess;
+	c->busy_set = false;
+	return hipStreamWaitEvent(c->stream, c->busy, 0);
 }
 
 inline int hip_fail(prt_ctx *c, hipError_t e, const char *what) {

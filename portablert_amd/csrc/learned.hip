@@ -288,6 +288,7 @@ int prt_mi355x_nearest_hits_learned_device(prt_ctx *c, const float *d_rays, uint
 	prt::TraceStats stats;
 	if (e == hipSuccess)
 		e = trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st, nullptr, c->tp.count_stats ? &stats : nullptr, &sa);
+	if (e == hipSuccess) e = mark_use(c, st);
 	if (e == hipSuccess && c->time_kernel) {
 		e = hipEventRecord(e2, st);
 		if (e == hipSuccess) e = hipEventSynchronize(e2);

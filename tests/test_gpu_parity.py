@@ -232,15 +232,28 @@ def test_bvh_invariants(gpu_ctx):
     dlo, dhi = dlo.astype(np.float32), dhi.astype(np.float32)
     u3 = np.repeat(used[:, None, :], 3, 1)
     assert (dlo[u3] <= lo[u3]).all() and (dhi[u3] >= hi[u3]).all()
+    # internal children: the parent's fp32 box of the child equals the union of the child
+    # node's own child boxes (both come from the same binary-tree boxes)
+    cbase = f[:, 0:4].copy().view(np.uint32)[:, 0]
+    inner = (meta & 0xE0) == 0xE0
+    node, slot = np.nonzero(inner)
+    child = cbase[node] + (meta[node, slot] & 0x1F)
+    assert (child < n_nodes).all()
+    cu = used[child]
+    clo = np.where(cu[:, None, :], lo[child], np.inf).min(axis=2)
+    chi = np.where(cu[:, None, :], hi[child], -np.inf).max(axis=2)
+    assert np.array_equal(clo, lo[node, :, slot]) and np.array_equal(chi, hi[node, :, slot])
     # leaves: boxes contain their triangles
     tbase = f[:, 4:8].copy().view(np.uint32)[:, 0]
     v = tr[:, :9].view(np.float32).reshape(-1, 3, 3)
     for k in range(8):
         m = meta[:, k]
         leaf = (m != 0) & ((m & 0xE0) != 0xE0)
-        cnt, off = m >> 5, m & 0x1F
-        for node in np.flatnonzero(leaf)[:5000]:
-            s = int(tbase[node]) + int(off[node])
-            tv = v[s:s + int(cnt[node])]
-            assert (tv.min(axis=(0, 1)) >= lo[node, :, k]).all()
-            assert (tv.max(axis=(0, 1)) <= hi[node, :, k]).all()
+        cnt, off = (m >> 5).astype(np.int64), (m & 0x1F).astype(np.int64)
+        nodes = np.flatnonzero(leaf)
+        first = tbase[nodes].astype(np.int64) + off[nodes]
+        for j in range(4):  # leaves hold 1..4 triangles
+            sel = cnt[nodes] > j
+            tv = v[first[sel] + j]
+            assert (tv.min(axis=1) >= lo[nodes[sel], :, k]).all()
+            assert (tv.max(axis=1) <= hi[nodes[sel], :, k]).all()
