@@ -27,39 +27,84 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_info():
+    model, smt = "unknown", None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count(), "smt": smt,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def cpu_baseline(tris, rays, target_s=10.0):
     """The oracle's restatement of the reference CPU backend (binned-SAH BVH2, depth-first
-    traversal, static thread split as intersect_cpu.hpp:20-43), timed on this host for about
-    `target_s` seconds: a prefix of the ray set sized from a short probe, repeated whole passes
-    of the full set when one pass is shorter than the target."""
+    traversal, static thread split as intersect_cpu.hpp:20-43), timed on this host: a prefix of
+    the ray set sized from a short probe so that each of 3 runs takes about target_s / 3 seconds
+    (repeated whole passes when one pass is shorter); the median run is reported."""
     import oracle
     threads = min(16, os.cpu_count() or 1)
     bvh = oracle.BVH2(tris)
-    n = 65536
+    n = min(len(rays), 1 << 20)  # large enough that thread start-up does not skew the probe
     t0 = time.perf_counter()
     bvh.trace(rays[:n], oracle.DFS, threads)
     rate = n / max(time.perf_counter() - t0, 1e-6)
-    want = int(rate * target_s)
+    want = int(rate * target_s / 3)
     reps = max(1, want // len(rays))
     count = min(len(rays), want) if reps == 1 else len(rays)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        bvh.trace(rays[:count], oracle.DFS, threads)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bvh.trace(rays[:count], oracle.DFS, threads)
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
     return {"value": round(reps * count / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} pass(es) over the first {count} of the 8,388,608 rays, same scene, "
-                      f"oracle depth-first mode (the reference CPU backend restated), {threads} threads, {dt:.1f} s"}
+            "sample": f"median of 3 runs, each {reps} pass(es) over the first {count} of the 8,388,608 rays, same "
+                      f"scene, oracle depth-first mode (the reference CPU backend restated), {threads} threads, "
+                      f"{dt:.2f} s per run",
+            "host": host_info()}
 
 
-def learned_leg(ctx, tris, steps, warmup, epochs):
+def fp64_report(tris, rays, rec):
+    """Error of fp32 hit records against a double-precision re-intersection of the same
+    (ray, triangle) pairs: for t the max relative error and the distance in fp32 ulps, for the
+    barycentrics u and v (in [0, 1]) the absolute error."""
+    import oracle
+    hit = np.isfinite(rec["t"]) & (rec["primitive_id"] != 0xFFFFFFFF)
+    if not hit.any():
+        return None
+    t64, u64, v64 = oracle.fp64(tris, rays[hit], rec["primitive_id"][hit])
+    out = {"hits": int(hit.sum())}
+    fields = [("t", t64)] + ([("u", u64), ("v", v64)] if "u" in rec.dtype.names else [])
+    for name, ref in fields:
+        got = rec[name][hit].astype(np.float64)
+        err = np.abs(got - ref)
+        if name == "t":  # relative to t, and in fp32 ulps of the result
+            ulp = err / np.spacing(np.abs(rec[name][hit])).astype(np.float64)
+            out[name] = {"max_rel": float((err / np.maximum(np.abs(ref), 1e-30)).max()),
+                         "max_ulp": float(ulp.max()), "p99_ulp": float(np.percentile(ulp, 99))}
+        else:  # barycentrics lie in [0, 1]: absolute error
+            out[name] = {"max_abs": float(err.max()), "p99_abs": float(np.percentile(err, 99))}
+    return out
+
+
+def learned_leg(ctx, tris, steps, warmup, epochs, train_rays=None):
     """Learned t_max seeding on the room scene: train on 16,777,216 rays traced by the plain
-    method, then time plain and learned-seeded traversal on 8,388,608 fresh rays (same records,
-    same rays). Training time is reported, not part of the traversal rate."""
+    method (or their first train_rays), then time plain and learned-seeded traversal on
+    8,388,608 fresh rays (same records, same rays). Training time is reported, not part of the
+    traversal rate."""
     import torch
 
     import portablert_amd as prt
     from portablert_amd import scenes
-    train = scenes.rays(4, stream=1)
+    train = scenes.rays(4, stream=1, count=train_rays)
     test = scenes.rays(4, stream=2)
     n = len(test)
     d_train = torch.from_numpy(train).cuda()
@@ -108,7 +153,7 @@ def learned_leg(ctx, tris, steps, warmup, epochs):
     ctx.set_option("count_stats", 0)
     flops = 2 * (192 * 256 + 2 * 256 * 256 + 256 * 4097) * n
     res = {
-        "rays": "room scene; 16,777,216 training rays, 8,388,608 fresh rays (uniform origins, uniform directions)",
+        "rays": f"room scene; {len(d_train):,} training rays, 8,388,608 fresh rays (uniform origins, uniform directions)",
         "epochs": epochs,
         "train_s": round(train_s, 2),
         "train_loss": round(ctx.stat("train_loss"), 4),
@@ -127,6 +172,37 @@ def learned_leg(ctx, tris, steps, warmup, epochs):
     }
     del d_train
     torch.cuda.empty_cache()
+    return res
+
+
+def precision_leg(ctx, d_rays, n, mask, d_ref, stream, steps):
+    """fp32 BVH8 nodes (208 B) against 8-bit quantized nodes (80 B) on the same BVH topology:
+    traversal time, records compared byte for byte, and counted node + triangle bytes per ray."""
+    import torch
+    sp = stream.cuda_stream
+    res = {}
+    fmt0 = ctx.get_option("node_format")
+    for fmt, name in ((1, "quantized8"), (0, "fp32")):
+        ctx.set_option("node_format", fmt)
+        out = torch.zeros_like(d_ref)
+        torch.cuda.synchronize()
+        ms = _timed(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out.data_ptr(), sp), steps, 1, stream)
+        ctx.set_option("count_stats", 1)
+        ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out.data_ptr(), sp)
+        ctx.synchronize()
+        ctx.set_option("count_stats", 0)
+        nodes, tris_t = ctx.stat("node_visits") / n, ctx.stat("tri_tests") / n
+        nb = ctx.stat("node_bytes")
+        res[name] = {"ms": round(ms, 4), "mrays_s": round(n / ms / 1e3, 2), "node_bytes": int(nb),
+                     "node_visits_per_ray": round(nodes, 3), "tri_tests_per_ray": round(tris_t, 3),
+                     "counted_bytes_per_ray": round(nodes * nb + tris_t * 48, 1),
+                     "identical_to_timed_run": bool(torch.equal(out, d_ref))}
+    ctx.set_option("node_format", fmt0)
+    q, f = res["quantized8"], res["fp32"]
+    res["node_byte_saving"] = round(1 - (q["node_visits_per_ray"] * q["node_bytes"]) /
+                                    (f["node_visits_per_ray"] * f["node_bytes"]), 4)
+    res["speedup_quantized"] = round(f["ms"] / q["ms"], 3)
+    log(f"precision: quantized {q['ms']:.3f} ms, fp32 {f['ms']:.3f} ms, node bytes saved {res['node_byte_saving']:.1%}")
     return res
 
 
@@ -249,6 +325,13 @@ def icosphere_leg(ctx, steps, warmup, stream):
         res[f"{name}_rays"] = len(r)
         res[f"{name}_ms"] = round(ms, 3)
         res[f"{name}_mrays_s"] = round(len(r) / ms / 1e3, 2)
+        rec = d_o.cpu().numpy().view(prt.hit_dtype(mask))
+        res[f"{name}_negative_t_rate"] = float(np.mean(rec["t"] < 0))
+        idx = np.sort(np.random.default_rng(2).choice(len(r), min(16384, len(r)), replace=False))
+        try:
+            res[f"{name}_fp64_error_sample"] = fp64_report(tris, r[idx], rec[idx])
+        except Exception as e:  # checker unavailable: report, do not fail the measurement
+            log("fp64 check skipped:", e)
     total = len(prim) + len(bounce)
     res["total_mrays_s"] = round(total / (res["primary_ms"] + res["bounce_ms"]) / 1e3, 2)
     log(f"icosphere: primary {res['primary_mrays_s']} Mrays/s, bounce {res['bounce_mrays_s']} Mrays/s")
@@ -267,6 +350,8 @@ def main():
                     help="also run a 512 MiB elementwise copy kernel (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding leg")
     ap.add_argument("--epochs", type=int, default=2, help="training epochs of the learned leg")
+    ap.add_argument("--train-rays", type=int, default=None,
+                    help="train the learned leg on the first N of the 16,777,216 training rays")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--no-icosphere", action="store_true", help="skip the icosphere primary + bounce leg")
     args = ap.parse_args()
@@ -339,24 +424,46 @@ def main():
         torch.cuda.synchronize()
         log("calibration: 3 x elementwise add over 512 MiB in, 512 MiB out")
 
-    # agreement of the timed output with the oracle on a sample (reported, not timed)
+    # agreement of the timed output with the oracle on a sample, and the error of that sample
+    # against a double-precision re-intersection (reported, not timed)
     hits = d_out.cpu().numpy().view(prt.hit_dtype(mask))
-    agree = None
+    agree, fp64 = None, None
     try:
         import oracle
-        idx = np.random.default_rng(1).choice(n, 16384, replace=False)
+        idx = np.sort(np.random.default_rng(1).choice(n, 16384, replace=False))
         want = oracle.trace(tris, rays[idx], oracle.LOWEST_ID, min(16, os.cpu_count() or 1))
         agree = float(np.mean((hits["t"][idx].view(np.uint32) == want["t"].view(np.uint32)) &
                               ((want["t"] == np.inf) | (hits["primitive_id"][idx] == want["primitive_id"]))))
+        fp64 = fp64_report(tris, rays[idx], hits[idx])
     except Exception as e:  # checker unavailable: report, do not fail the measurement
         log("oracle check skipped:", e)
+    neg_t = float(np.mean(hits["t"] < 0))
+
+    # the host-buffer entry point: PCIe copies of rays in and records out included
+    api_ms = []
+    for _ in range(3):
+        ctx.nearest_hits(rays, mask)
+        api_ms.append(ctx.stat("api_ms"))
+    api_ms = float(np.median(api_ms))
+
+    # precision: fp32 BVH8 nodes against 8-bit quantized nodes, same records
+    precision = precision_leg(ctx, d_rays, n, mask, d_out, stream, max(5, args.steps // 2))
 
     achieved = BYTES_PER_RAY * n / (kernel_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, valu = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_room.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    vf = os.path.join(ROOT, "profiles", "valu_room.json")
+    if os.path.exists(vf) and args.format == 1:
+        # the bound that holds: VALU issue (instructions per launch from the committed counters)
+        with open(vf) as f:
+            vj = json.load(f)
+        rate = vj["sq_insts_valu_per_dispatch"] / (kernel_ms * 1e-3)
+        valu = {"bound": "valu_issue", "achieved": round(rate / 1e9, 1), "peak": round(vj["peak_wave_insts_per_s"] / 1e9, 1),
+                "unit": "G wave-instructions/s", "frac": round(rate / vj["peak_wave_insts_per_s"], 4),
+                "lane_activity": vj["lane_activity"]}
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -377,15 +484,21 @@ def main():
                    "traversal": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
+        "valu_roofline": valu,
         "build_ms": round(build_ms, 3),
         "kernel_ms": round(kernel_ms, 4),
         "traversal_kernel_ms": round(lib_kernel_ms, 4),
         "plain_mrays_s": round(value, 2),
         "learned_mrays_s": None,
         "oracle_agreement_sample": agree,
+        "fp64_error_sample": fp64,
+        "negative_t_rate": neg_t,
+        "api_boundary_ms": round(api_ms, 3),
+        "api_boundary_mrays_s": round(n / api_ms / 1e3, 2),
+        "precision": precision,
     }
     if not args.no_learned:
-        lr = learned_leg(ctx, tris, max(3, args.steps // 4), 1, args.epochs)
+        lr = learned_leg(ctx, tris, max(3, args.steps // 4), 1, args.epochs, args.train_rays)
         out["learned_mrays_s"] = lr["learned_mrays_s"]
         out["learned"] = lr
     if not args.no_batch:

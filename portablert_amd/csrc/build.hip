@@ -89,12 +89,20 @@ __device__ __forceinline__ void scene_atomics(uint32_t *w, const float c[3], con
 	}
 }
 
+constexpr int kPrepBlock = 1024;
+
 // Triangle boxes (the scene index in the w word of the lower corner) and per-scene centroid and
-// vertex bounds: one set of atomics per wave when the wave lies in one scene.
-__global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ tris, uint32_t n,
-                                                     const uint32_t *__restrict__ off, uint32_t scenes,
-                                                     float4 *__restrict__ tbox, uint32_t *misc) {
-	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// vertex bounds. The bounds are reduced over the workgroup before the atomics when all of its
+// triangles belong to one scene (one set of 12 atomics per 1024 triangles: same-address
+// atomics serialise), else per lane.
+__global__ __launch_bounds__(kPrepBlock) void k_tri_prep(const float *__restrict__ tris, uint32_t n,
+                                                         const uint32_t *__restrict__ off, uint32_t scenes,
+                                                         float4 *__restrict__ tbox, uint32_t *misc) {
+	__shared__ float red[kPrepBlock / 64][12];
+	__shared__ uint32_t mixed;
+	const uint32_t i = blockIdx.x * kPrepBlock + threadIdx.x;
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	if (threadIdx.x == 0) mixed = 0u;
 	float c[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 	float cmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	float smn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
@@ -120,13 +128,18 @@ __global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ t
 		tbox[2ull * i] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(scene));
 		tbox[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
 	}
-	// lanes past n take the scene of lane 0 with empty bounds
-	const uint32_t s0 = __shfl(scene, 0);
-	if (__ballot(i < n && scene != s0) != 0ull) {
+	// the block's first triangle decides the scene the block reduces for
+	const uint32_t first = blockIdx.x * kPrepBlock;
+	const uint32_t s0 = scenes > 1 ? scene_of(off, scenes, first) : 0u;
+	__syncthreads();
+	if (i < n && scene != s0) mixed = 1u;
+	__syncthreads();
+	const bool block_mixed = mixed != 0u;  // the block spans a scene boundary (rare)
+	const uint32_t ws = __shfl(scene, 0);
+	if (block_mixed && __ballot(i < n && scene != ws) != 0ull) {  // this wave spans it: per lane
 		if (i < n) scene_atomics(misc + M_SCENE + M_PER_SCENE * scene, c, cmax, smn, smx);
 		return;
 	}
-	// wave reduction, then one atomic per wave and bound
 #pragma unroll
 	for (int k = 0; k < 3; ++k) {
 		for (int sh = 32; sh > 0; sh >>= 1) {
@@ -136,7 +149,31 @@ __global__ __launch_bounds__(kBlock) void k_tri_prep(const float *__restrict__ t
 			smx[k] = fmaxf(smx[k], __shfl_xor(smx[k], sh));
 		}
 	}
-	if ((threadIdx.x & 63) == 0 && c[0] <= cmax[0]) scene_atomics(misc + M_SCENE + M_PER_SCENE * s0, c, cmax, smn, smx);
+	if (block_mixed) {  // uniform wave in a mixed block: one set of atomics per wave
+		if (lane == 0 && !__builtin_isnan(c[0])) scene_atomics(misc + M_SCENE + M_PER_SCENE * ws, c, cmax, smn, smx);
+		return;
+	}
+	if (lane == 0) {
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			red[wave][k] = c[k];
+			red[wave][3 + k] = cmax[k];
+			red[wave][6 + k] = smn[k];
+			red[wave][9 + k] = smx[k];
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 12) {
+		const uint32_t k = threadIdx.x;
+		const bool is_min = ((k / 3) & 1u) == 0u;  // centroid min, centroid max, box min, box max
+		float r = red[0][k];
+		for (int w = 1; w < kPrepBlock / 64; ++w) r = is_min ? fminf(r, red[w][k]) : fmaxf(r, red[w][k]);
+		if (!__builtin_isnan(r)) {  // lanes past n contribute the identities +-inf
+			uint32_t *dst = misc + M_SCENE + M_PER_SCENE * s0 + k;
+			if (is_min) atomicMin(dst, f2o(r));
+			else atomicMax(dst, f2o(r));
+		}
+	}
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {
@@ -188,7 +225,40 @@ __global__ __launch_bounds__(kBlock) void k_hist(const uint64_t *__restrict__ ke
 	hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of hist (digit-major: entry d * tiles + t) in place, one workgroup.
+constexpr int kScanItems = 16;                 // histogram entries per thread
+constexpr int kScanSeg = kBlock * kScanItems;  // entries per segment
+
+// Exclusive scan of hist (digit-major: entry d * tiles + t) in two levels: each segment of
+// 4096 entries is scanned in place by one workgroup and its total goes to seg[]; k_scan then
+// scans seg[] and the scatter adds seg[entry / 4096].
+__global__ __launch_bounds__(kBlock) void k_scan_seg(uint32_t *hist, uint32_t total, uint32_t *seg) {
+	__shared__ uint32_t part[kBlock];
+	const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanSeg + static_cast<uint64_t>(threadIdx.x) * kScanItems;
+	uint32_t v[kScanItems];
+	uint32_t s = 0;
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k) {
+		v[k] = base + k < total ? hist[base + k] : 0u;
+		s += v[k];
+	}
+	part[threadIdx.x] = s;
+	__syncthreads();
+	for (int sh = 1; sh < kBlock; sh <<= 1) {
+		const uint32_t a = threadIdx.x >= static_cast<uint32_t>(sh) ? part[threadIdx.x - sh] : 0u;
+		__syncthreads();
+		part[threadIdx.x] += a;
+		__syncthreads();
+	}
+	uint32_t run = part[threadIdx.x] - s;
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k) {
+		if (base + k < total) hist[base + k] = run;
+		run += v[k];
+	}
+	if (threadIdx.x == kBlock - 1) seg[blockIdx.x] = part[threadIdx.x];
+}
+
+// Exclusive scan of a short array in place, one workgroup.
 __global__ __launch_bounds__(1024) void k_scan(uint32_t *hist, uint32_t total) {
 	__shared__ uint32_t part[1024];
 	const uint32_t per = (total + 1023) / 1024;
@@ -217,11 +287,12 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t *hist, uint32_t total) {
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                     uint32_t n, int shift, const uint32_t *__restrict__ hist,
-                                                    uint32_t tiles) {
+                                                    const uint32_t *__restrict__ seg, uint32_t tiles) {
 	__shared__ uint32_t offs[256];
 	__shared__ uint32_t wcnt[kBlock / 64][256];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	offs[threadIdx.x] = hist[static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x];
+	const uint64_t h = static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x;
+	offs[threadIdx.x] = hist[h] + seg[h / kScanSeg];
 	const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 	for (int r = 0; r < kSortItems; ++r) {
@@ -673,7 +744,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		PRT_TRY(s.vals[k].ensure(sizeof(uint32_t) * n));
 	}
 	const uint32_t tiles = blocks_for(n, kSortTile);
-	PRT_TRY(s.hist.ensure(sizeof(uint32_t) * 256 * tiles));
+	const uint32_t hist_n = 256 * tiles, n_seg = blocks_for(hist_n, kScanSeg);
+	PRT_TRY(s.hist.ensure(sizeof(uint32_t) * (static_cast<uint64_t>(hist_n) + n_seg)));
 	PRT_TRY(s.bin.ensure(sizeof(uint32_t) * (6ull * nb + 8)));
 	PRT_TRY(s.bbox.ensure(sizeof(float4) * 2ull * nb));
 	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * cap));
@@ -688,7 +760,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	uint32_t *misc = s.misc.as<uint32_t>();
 	const float *tri_in = ss.tris ? ss.tris : s.tri_in.as<float>();
 	hipLaunchKernelGGL(k_init_misc, dim3(blocks_for(words)), dim3(kBlock), 0, st, misc, words);
-	hipLaunchKernelGGL(k_tri_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, n, off, S, s.tbox.as<float4>(),
+	hipLaunchKernelGGL(k_tri_prep, dim3(blocks_for(n, kPrepBlock)), dim3(kPrepBlock), 0, st, tri_in, n, off, S, s.tbox.as<float4>(),
 	                   misc);
 	hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(), n, misc,
 	                   s.keys[0].as<uint64_t>(), s.vals[0].as<uint32_t>());
@@ -698,10 +770,12 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	for (int shift = 0; shift < key_bits; shift += 8) {
 		hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(), n, shift,
 		                   s.hist.as<uint32_t>(), tiles);
-		hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.hist.as<uint32_t>(), 256 * tiles);
+		uint32_t *seg = s.hist.as<uint32_t>() + hist_n;
+		hipLaunchKernelGGL(k_scan_seg, dim3(n_seg), dim3(kBlock), 0, st, s.hist.as<uint32_t>(), hist_n, seg);
+		hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, seg, n_seg);
 		hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kBlock), 0, st, s.keys[cur].as<uint64_t>(),
 		                   s.vals[cur].as<uint32_t>(), s.keys[cur ^ 1].as<uint64_t>(), s.vals[cur ^ 1].as<uint32_t>(),
-		                   n, shift, s.hist.as<uint32_t>(), tiles);
+		                   n, shift, s.hist.as<uint32_t>(), seg, tiles);
 		cur ^= 1;
 	}
 	const uint64_t *keys = s.keys[cur].as<uint64_t>();

@@ -22,7 +22,7 @@ struct Mlp {
 	// parameters: fp32 master W^T [out][in] and bias, bf16 W^T and W, Adam moments, gradients
 	DevBuf w[4], b[4], wt_bf[4], w_bf[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4];
 	// activations and work buffers (grown to the largest batch)
-	DevBuf x0, h[3], logits, dlog, dlog_t, dh[3], act_t, dact_t, split, loss, top_part;
+	DevBuf x0, h[3], logits, dlog, dlog_t, dh[3], act_t, dact_t, split, loss;
 };
 
 hipError_t mlp_init(Mlp &m, uint64_t seed, hipStream_t st);

@@ -24,9 +24,11 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBM = 128, kBN = 128, kBK = 32, kThreads = 256;
 constexpr int kCPad = 132;  // fp32 row stride of the epilogue tile in LDS
+constexpr size_t kSplitFloats = static_cast<size_t>(8) * kMlpOutPad * kMlpHidden;  // split-K / column-sum scratch
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
 	// round to nearest even (inputs are finite here)
@@ -40,19 +42,17 @@ enum Epi : int {
 	E_BIAS_RELU_BF16 = 0,  // out bf16 = relu(acc + bias[n])
 	E_BIAS_F32 = 1,        // out f32 = acc + bias[n]
 	E_MASK_BF16 = 2,       // out bf16 = acc * (aux_bf16[m][n] > 0)
-	E_F32 = 3,             // out f32 = acc (split-K partial: out += split * M * N)
-	E_TOP4 = 4             // per row: top-4 of (acc + bias) over this column tile -> out (val f32, idx u32)
+	E_F32 = 3              // out f32 = acc (split-K partial: out += split * M * N)
 };
 
 struct GemmArgs {
 	const uint16_t *A;   // [M][lda] bf16
 	const uint16_t *B;   // [N][ldb] bf16
 	void *out;
-	const float *bias;   // [N] (E_BIAS_*, E_TOP4)
+	const float *bias;   // [N] (E_BIAS_*)
 	const uint16_t *aux; // [M][ldo] bf16 (E_MASK_BF16)
 	int M, N, K, lda, ldb, ldo;
 	int k_split;         // K range per z-slice (split-K), multiple of kBK
-	int n_valid;         // columns >= n_valid are padding (E_TOP4 ignores them)
 };
 
 // C tile (128 x 128) of one workgroup: 4 waves, each 64 x 64 as 4 x 4 MFMA 16x16 tiles.
@@ -108,53 +108,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmArgs g) {
 				Cs[(wr + i * 16 + (lane >> 4) * 4 + e) * kCPad + wc + j * 16 + (lane & 15)] = acc[i][j][e];
 	__syncthreads();
 
-	if constexpr (EPI == E_TOP4) {
-		// two threads per row, 64 columns each, then a pairwise merge
-		const int row = tid >> 1, half = tid & 1;
-		float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-		uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-		for (int c = half * 64; c < half * 64 + 64; ++c) {
-			const int n = n0 + c;
-			if (n >= g.n_valid) break;
-			const float x = Cs[row * kCPad + c] + g.bias[n];
-			if (x > v[3]) {  // insert, keeping v sorted descending (ties keep the lower index first)
-				v[3] = x;
-				ix[3] = static_cast<uint32_t>(n);
-#pragma unroll
-				for (int q = 3; q > 0; --q) {
-					if (v[q] > v[q - 1]) {
-						const float tv = v[q]; v[q] = v[q - 1]; v[q - 1] = tv;
-						const uint32_t ti = ix[q]; ix[q] = ix[q - 1]; ix[q - 1] = ti;
-					}
-				}
-			}
-		}
-		float ov[4];
-		uint32_t oi[4];
-#pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			ov[q] = __shfl_xor(v[q], 1);
-			oi[q] = __shfl_xor(ix[q], 1);
-		}
-		if (half == 0 && m0 + row < g.M) {
-			// merge two descending lists (the partner holds the higher column half)
-			float rv[4];
-			uint32_t ri[4];
-			int a = 0, b = 0;
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				const bool take_a = v[a] >= ov[b];
-				rv[q] = take_a ? v[a] : ov[b];
-				ri[q] = take_a ? ix[a] : oi[b];
-				a += take_a ? 1 : 0;
-				b += take_a ? 0 : 1;
-			}
-			float2 *o = reinterpret_cast<float2 *>(g.out) + (static_cast<size_t>(m0 + row) * gridDim.x + blockIdx.x) * 4;
-#pragma unroll
-			for (int q = 0; q < 4; ++q) o[q] = make_float2(rv[q], __uint_as_float(ri[q]));
-		}
-		return;
-	}
 	// element-wise epilogues: each thread handles 64 consecutive columns... of half a row
 	const int row = tid >> 1, cbase = (tid & 1) * 64;
 	const int m = m0 + row;
@@ -177,13 +130,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmArgs g) {
 }
 
 // Positional encoding: 6 inputs (origin scaled to [-1,1] by the scene box, direction) times
-// 16 frequencies 2^(j/2) * pi, sin and cos: 192 features, written as bf16 [n][192] in the
-// order (input, frequency, sin/cos).
-__global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, uint64_t first, uint32_t n, float4 lo,
-                                                float4 inv_ext, uint16_t *__restrict__ x) {
-	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= n) return;
-	const float *r = rays + 6 * (first + i);
+// 16 frequencies 2^(j/2) * pi, sin and cos: 192 features, written as bf16 in the order
+// (input, frequency, sin/cos). encode_inputs writes the features of inputs [a0, a0 + 3).
+__device__ __forceinline__ void encode_inputs(const float *r, float4 lo, float4 inv_ext, int a0, uint16_t *o) {
 	float in[6];
 	in[0] = (r[0] - lo.x) * inv_ext.x * 2.0f - 1.0f;
 	in[1] = (r[1] - lo.y) * inv_ext.y * 2.0f - 1.0f;
@@ -191,16 +140,249 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 	in[3] = r[3];
 	in[4] = r[4];
 	in[5] = r[5];
-	uint16_t *o = x + static_cast<size_t>(i) * kMlpIn;
 #pragma unroll
-	for (int a = 0; a < 6; ++a) {
+	for (int q = 0; q < 3; ++q) {
+		const int a = a0 + q;
+		const float x = a0 == 0 ? in[q] : in[3 + q];
 #pragma unroll
 		for (int j = 0; j < 16; ++j) {
 			// v_sin/v_cos take revolutions: sin(2 pi y); y = in * 2^(j/2) / 2
-			const float y = in[a] * (0.5f * exp2f(0.5f * static_cast<float>(j)));
+			const float y = x * (0.5f * exp2f(0.5f * static_cast<float>(j)));
 			o[a * 32 + j * 2] = f2bf(__builtin_amdgcn_sinf(y));
 			o[a * 32 + j * 2 + 1] = f2bf(__builtin_amdgcn_cosf(y));
 		}
+	}
+}
+
+__global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, uint64_t first, uint32_t n, float4 lo,
+                                                float4 inv_ext, uint16_t *__restrict__ x) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	const float *r = rays + 6 * (first + i);
+	uint16_t *o = x + static_cast<size_t>(i) * kMlpIn;
+	encode_inputs(r, lo, inv_ext, 0, o);
+	encode_inputs(r, lo, inv_ext, 3, o);
+}
+
+// ------------------------------------------------------------- fused inference with top-4
+//
+// One workgroup (4 waves) takes 128 rays through the whole network: encoding, the three hidden
+// layers and the 4097 output logits, keeping the activations in LDS (X) and streaming W^T tiles
+// of 128 output neurons (W, prefetched into registers one tile ahead) from L2. Every 128 x 128
+// tile is the same sequence of v_mfma_f32_16x16x32_bf16 over K in steps of 32 as k_gemm, so
+// the logits are bit-identical to the unfused path; the output tiles go through LDS (over W)
+// to two row-owner threads per ray that keep a running top-4 (value descending, index
+// ascending) across all 33 tiles, so no logit leaves the chip.
+constexpr int kFRows = 128;    // rays per workgroup
+constexpr int kFStride = 264;  // bf16 row stride of X and W (16-byte reads without bank conflicts)
+constexpr int kFCStride = 132; // fp32 row stride of the output tile over W
+
+struct FusedArgs {
+	const float *rays;
+	uint64_t first;
+	int rows;
+	float4 lo, inv_ext;
+	const uint16_t *w[4];  // W^T bf16 [out][in]
+	const float *b[4];
+	uint32_t *top4;        // [rows][4]
+};
+
+// Global -> registers: the 128 x K bf16 tile of W^T rows [n0, n0 + 128), 16-byte chunks,
+// chunk c = tid + 256 q is row c / (K / 8), column group c % (K / 8).
+template <int K>
+__device__ __forceinline__ void tile_fetch(const uint16_t *wt, int n0, int tid, u32x4 pre[K / 16]) {
+	constexpr int per_row = K / 8;
+#pragma unroll
+	for (int q = 0; q < K / 16; ++q) {
+		const int c = tid + 256 * q;
+		pre[q] = *reinterpret_cast<const u32x4 *>(wt + static_cast<size_t>(n0 + c / per_row) * K + (c % per_row) * 8);
+	}
+}
+
+template <int K>
+__device__ __forceinline__ void tile_store(uint16_t *W, int tid, const u32x4 pre[K / 16]) {
+	constexpr int per_row = K / 8;
+#pragma unroll
+	for (int q = 0; q < K / 16; ++q) {
+		const int c = tid + 256 * q;
+		*reinterpret_cast<u32x4 *>(W + (c / per_row) * kFStride + (c % per_row) * 8) = pre[q];
+	}
+}
+
+// acc (this wave's 64 x 64 part at rows wr, columns wc) = X[:, :K] * W[:, :K]^T
+template <int K>
+__device__ __forceinline__ void tile_mma(const uint16_t *X, const uint16_t *W, int wr, int wc, int lane,
+                                         f32x4 acc[4][4]) {
+#pragma unroll
+	for (int i = 0; i < 4; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+	for (int k0 = 0; k0 < K; k0 += kBK) {
+		bf16x8 af[4], bfr[4];
+#pragma unroll
+		for (int s = 0; s < 4; ++s) {
+			af[s] = *reinterpret_cast<const bf16x8 *>(X + (wr + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
+			bfr[s] = *reinterpret_cast<const bf16x8 *>(W + (wc + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
+		}
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+	}
+}
+
+// X[row][n0 + col] = bf16(relu(acc + bias)) for this wave's part of a 128 x 128 tile
+__device__ __forceinline__ void tile_relu_store(uint16_t *X, const f32x4 acc[4][4], const float *bias, int n0, int wr,
+                                                int wc, int lane) {
+#pragma unroll
+	for (int j = 0; j < 4; ++j) {
+		const int col = n0 + wc + j * 16 + (lane & 15);
+		const float bj = bias[col];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int e = 0; e < 4; ++e)
+				X[(wr + i * 16 + (lane >> 4) * 4 + e) * kFStride + col] = f2bf(fmaxf(acc[i][j][e] + bj, 0.0f));
+	}
+}
+
+// One hidden layer over the 128 rays of X (input width K, 256 outputs).
+template <int K>
+__device__ __forceinline__ void hidden_layer(uint16_t *X, uint16_t *W, const uint16_t *wt, const float *bias, int tid,
+                                             int wr, int wc, int lane) {
+	f32x4 acc0[4][4], acc1[4][4];
+	{
+		u32x4 pre[K / 16];
+		tile_fetch<K>(wt, 0, tid, pre);
+		tile_store<K>(W, tid, pre);
+	}
+	__syncthreads();
+	tile_mma<K>(X, W, wr, wc, lane, acc0);
+	__syncthreads();
+	{
+		u32x4 pre[K / 16];
+		tile_fetch<K>(wt, 128, tid, pre);
+		tile_store<K>(W, tid, pre);
+	}
+	__syncthreads();
+	tile_mma<K>(X, W, wr, wc, lane, acc1);
+	__syncthreads();
+	tile_relu_store(X, acc0, bias, 0, wr, wc, lane);
+	tile_relu_store(X, acc1, bias, 128, wr, wc, lane);
+	__syncthreads();
+}
+
+__device__ __forceinline__ bool top_better(float va, uint32_t ia, float vb, uint32_t ib) {
+	return va > vb || (va == vb && ia < ib);
+}
+
+// Inserts (x, n) into a list of 4 kept in (value descending, index ascending) order.
+__device__ __forceinline__ void top_insert(float v[4], uint32_t ix[4], float x, uint32_t n) {
+	if (!top_better(x, n, v[3], ix[3])) return;
+	v[3] = x;
+	ix[3] = n;
+#pragma unroll
+	for (int k = 3; k > 0; --k) {
+		if (top_better(v[k], ix[k], v[k - 1], ix[k - 1])) {
+			const float tv = v[k]; v[k] = v[k - 1]; v[k - 1] = tv;
+			const uint32_t ti = ix[k]; ix[k] = ix[k - 1]; ix[k - 1] = ti;
+		}
+	}
+}
+
+__global__ __launch_bounds__(256, 1) void k_mlp_top4(FusedArgs a) {
+	__shared__ __align__(16) uint16_t X[kFRows * kFStride];
+	__shared__ __align__(16) uint16_t W[kFRows * kFStride];
+	float *C = reinterpret_cast<float *>(W);  // output tile [128][132] fp32, over W
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+	const int r0 = blockIdx.x * kFRows;
+	// encoding: two threads per ray, three inputs each; rows past the end are zero
+	{
+		const int row = tid >> 1, half = tid & 1;
+		uint16_t *o = X + row * kFStride;
+		if (r0 + row < a.rows) {
+			encode_inputs(a.rays + 6 * (a.first + r0 + row), a.lo, a.inv_ext, half * 3, o);
+		} else {
+			for (int c = half * 96; c < half * 96 + 96; ++c) o[c] = 0;
+		}
+	}
+	// hidden layers: two tiles of 128 neurons each (loaded without prefetch, so only the two
+	// accumulator sets are live), then X is overwritten with the layer output
+	hidden_layer<kMlpIn>(X, W, a.w[0], a.b[0], tid, wr, wc, lane);
+	hidden_layer<kMlpHidden>(X, W, a.w[1], a.b[1], tid, wr, wc, lane);
+	hidden_layer<kMlpHidden>(X, W, a.w[2], a.b[2], tid, wr, wc, lane);
+	{
+		u32x4 p0[kMlpHidden / 16];
+		tile_fetch<kMlpHidden>(a.w[3], 0, tid, p0);
+		tile_store<kMlpHidden>(W, tid, p0);
+	}
+	__syncthreads();
+	// Output layer: each tile (plus bias) goes through LDS (over W) to two row-owner threads per
+	// ray (columns 0-63 and 64-127), which keep a running top-4. A thread sees its columns in
+	// ascending order, so a strict comparison keeps the lower index first among equal logits.
+	const int orow = tid >> 1, ohalf = tid & 1;
+	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+	constexpr int kOutTiles = kMlpOutPad / 128;
+	for (int t = 0; t < kOutTiles; ++t) {
+		// next tile (the first again after the last: unconditional loads keep pre in registers);
+		// the loads overlap this tile's MFMAs and the top-4 scan
+		u32x4 pre[kMlpHidden / 16];
+		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * 128, tid, pre);
+		f32x4 acc[4][4];
+		tile_mma<kMlpHidden>(X, W, wr, wc, lane, acc);
+		__syncthreads();  // every wave is done reading W
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const int col = wc + j * 16 + (lane & 15);
+			const float bj = a.b[3][t * 128 + col];
+#pragma unroll
+			for (int i = 0; i < 4; ++i)
+#pragma unroll
+				for (int e = 0; e < 4; ++e) C[(wr + i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][j][e] + bj;
+		}
+		__syncthreads();
+		const float *crow = C + orow * kFCStride + ohalf * 64;
+		const int nbase = t * 128 + ohalf * 64;
+		for (int c4 = 0; c4 < 16; ++c4) {
+			const float4 q = *reinterpret_cast<const float4 *>(crow + c4 * 4);
+			const float xs[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				const int n = nbase + c4 * 4 + e;
+				if (xs[e] > v[3] && n < kMlpOut) {
+					v[3] = xs[e];
+					ix[3] = static_cast<uint32_t>(n);
+#pragma unroll
+					for (int k = 3; k > 0; --k) {
+						if (v[k] > v[k - 1]) {
+							const float tv = v[k]; v[k] = v[k - 1]; v[k - 1] = tv;
+							const uint32_t ti = ix[k]; ix[k] = ix[k - 1]; ix[k - 1] = ti;
+						}
+					}
+				}
+			}
+		}
+		__syncthreads();  // the tile in LDS is read
+		tile_store<kMlpHidden>(W, tid, pre);
+		__syncthreads();
+	}
+	// merge the two halves of each ray (value descending, index ascending): insert the
+	// partner's four entries into this list
+	float ov[4];
+	uint32_t oi[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		ov[q] = __shfl_xor(v[q], 1);
+		oi[q] = __shfl_xor(ix[q], 1);
+	}
+#pragma unroll
+	for (int q = 0; q < 4; ++q) top_insert(v, ix, ov[q], oi[q]);
+	if (ohalf == 0 && r0 + orow < a.rows) {
+		const uint32_t res[4] = {ix[0], ix[1], ix[2], ix[3]};
+		reinterpret_cast<uint4 *>(a.top4)[r0 + orow] = make_uint4(res[0], res[1], res[2], res[3]);
 	}
 }
 
@@ -270,14 +452,17 @@ __global__ __launch_bounds__(256) void k_reduce_split(const float *__restrict__ 
 	g[i] = s;
 }
 
-// Column sums of a bf16 [rows][ld] matrix over its first cols columns (bias gradient).
-__global__ __launch_bounds__(256) void k_colsum(const uint16_t *__restrict__ d, int rows, int cols, int ld,
-                                                float *__restrict__ g) {
+// Column sums of a bf16 [rows][ld] matrix over its first cols columns (bias gradient), first
+// stage: part[chunk][c] = sum of rows [chunk * per, (chunk + 1) * per) of column c. The second
+// stage (k_reduce_split) adds the chunks in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum_part(const uint16_t *__restrict__ d, int rows, int cols, int ld, int per,
+                                                     float *__restrict__ part) {
 	const int c = blockIdx.x * 256 + threadIdx.x;
 	if (c >= cols) return;
+	const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
 	float s = 0.0f;
-	for (int r = 0; r < rows; ++r) s += bf2f(d[static_cast<size_t>(r) * ld + c]);
-	g[c] = s;
+	for (int r = r0; r < r1; ++r) s += bf2f(d[static_cast<size_t>(r) * ld + c]);
+	part[static_cast<size_t>(blockIdx.y) * cols + c] = s;
 }
 
 // Adam on fp32 master parameters; refreshes the bf16 copy (W^T [rows][cols]) and, when wt is
@@ -303,33 +488,6 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float
 			bft[static_cast<size_t>(c) * rows + r] = h;
 		}
 	}
-}
-
-// Merges per-tile top-4 lists into the top-4 clusters of each ray.
-__global__ __launch_bounds__(256) void k_top4_merge(const float2 *__restrict__ part, int rows, int tiles,
-                                                    uint32_t *__restrict__ top) {
-	const int r = blockIdx.x * 256 + threadIdx.x;
-	if (r >= rows) return;
-	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-	const float2 *p = part + static_cast<size_t>(r) * tiles * 4;
-	for (int k = 0; k < tiles * 4; ++k) {
-		const float x = p[k].x;
-		const uint32_t id = __float_as_uint(p[k].y);
-		if (x > v[3] || (x == v[3] && id < ix[3])) {
-			v[3] = x;
-			ix[3] = id;
-#pragma unroll
-			for (int q = 3; q > 0; --q) {
-				if (v[q] > v[q - 1] || (v[q] == v[q - 1] && ix[q] < ix[q - 1])) {
-					const float tv = v[q]; v[q] = v[q - 1]; v[q - 1] = tv;
-					const uint32_t ti = ix[q]; ix[q] = ix[q - 1]; ix[q - 1] = ti;
-				}
-			}
-		}
-	}
-	uint4 o = make_uint4(ix[0], ix[1], ix[2], ix[3]);
-	reinterpret_cast<uint4 *>(top)[r] = o;
 }
 
 // Deterministic initialisation (uniform +-sqrt(6/(fan_in+fan_out)) from a counter hash).
@@ -436,11 +594,8 @@ static hipError_t ensure_act(Mlp &m, int rows, bool train) {
 		for (int l = 0; l < 3; ++l) PRT_TRY(m.dh[l].ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
 		PRT_TRY(m.act_t.ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
 		PRT_TRY(m.dact_t.ensure(static_cast<size_t>(rows) * kMlpHidden * 2));
-		PRT_TRY(m.split.ensure(static_cast<size_t>(8) * kMlpOutPad * kMlpHidden * 4));
+		PRT_TRY(m.split.ensure(kSplitFloats * 4));
 		PRT_TRY(m.loss.ensure(4));
-	} else {
-		const int tiles = kMlpOutPad / kBN;
-		PRT_TRY(m.top_part.ensure(static_cast<size_t>(rows) * tiles * 4 * 8));
 	}
 	return hipSuccess;
 }
@@ -495,24 +650,19 @@ hipError_t mlp_forward_logits(Mlp &m, int rows, hipStream_t st) {
 }
 
 hipError_t mlp_infer_top4(Mlp &m, const float *rays, uint64_t first, int rows, uint32_t *top4, hipStream_t st) {
-	PRT_TRY(ensure_act(m, rows, false));
-	PRT_TRY(mlp_encode(m, rays, first, rows, st));
-	PRT_TRY(forward_hidden(m, rows, st));
-	GemmArgs g{};
-	g.A = m.h[2].as<uint16_t>();
-	g.B = m.wt_bf[3].as<uint16_t>();
-	g.out = m.top_part.ptr;
-	g.bias = m.b[3].as<float>();
-	g.M = rows;
-	g.N = kMlpOutPad;
-	g.K = kMlpHidden;
-	g.lda = g.ldb = kMlpHidden;
-	g.ldo = kMlpOutPad;
-	g.k_split = g.K;
-	g.n_valid = kMlpOut;
-	PRT_TRY(gemm<E_TOP4>(g, 1, st));
-	hipLaunchKernelGGL(k_top4_merge, dim3(blocks(rows)), dim3(256), 0, st, m.top_part.as<float2>(), rows,
-	                   kMlpOutPad / kBN, top4);
+	if (rows <= 0) return hipSuccess;
+	FusedArgs f;
+	f.rays = rays;
+	f.first = first;
+	f.rows = rows;
+	f.lo = make_float4(m.scene_lo[0], m.scene_lo[1], m.scene_lo[2], 0.f);
+	f.inv_ext = make_float4(m.scene_inv_ext[0], m.scene_inv_ext[1], m.scene_inv_ext[2], 0.f);
+	for (int l = 0; l < 4; ++l) {
+		f.w[l] = m.wt_bf[l].as<uint16_t>();
+		f.b[l] = m.b[l].as<float>();
+	}
+	f.top4 = top4;
+	hipLaunchKernelGGL(k_mlp_top4, dim3((rows + kFRows - 1) / kFRows), dim3(256), 0, st, f);
 	return hipGetLastError();
 }
 
@@ -527,8 +677,15 @@ hipError_t mlp_backward(Mlp &m, const uint32_t *labels, int rows, hipStream_t st
 	for (int l = 3; l >= 0; --l) {
 		const int in = kLayerIn[l], out = kLayerOut[l];
 		const uint16_t *x = l == 0 ? m.x0.as<uint16_t>() : m.h[l - 1].as<uint16_t>();
-		// bias gradient: column sums of dy
-		hipLaunchKernelGGL(k_colsum, dim3(blocks(out)), dim3(256), 0, st, dy, rows, out, dy_cols, m.gb[l].as<float>());
+		// bias gradient: column sums of dy (chunks of >= 128 rows whose partial sums fit the
+		// split-K scratch, then the chunks in order)
+		const int max_chunks = static_cast<int>(std::min<size_t>(65535, kSplitFloats / static_cast<size_t>(out)));
+		const int per = std::max(128, (rows + max_chunks - 1) / max_chunks);
+		const int chunks = (rows + per - 1) / per;
+		hipLaunchKernelGGL(k_colsum_part, dim3(blocks(out), chunks), dim3(256), 0, st, dy, rows, out, dy_cols, per,
+		                   m.split.as<float>());
+		hipLaunchKernelGGL(k_reduce_split, dim3(blocks(out)), dim3(256), 0, st, m.split.as<float>(), chunks,
+		                   static_cast<size_t>(out), m.gb[l].as<float>());
 		// weight gradient gW^T[out][in] = dy^T[out][rows] * x^T[in][rows]^T (split over the batch)
 		uint16_t *dyt = (l == 3) ? m.dlog_t.as<uint16_t>() : m.dact_t.as<uint16_t>();
 		hipLaunchKernelGGL(k_transpose, dim3((out + 31) / 32, (rows + 31) / 32), dim3(256), 0, st, dy, rows, out, dy_cols,
