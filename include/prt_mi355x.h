@@ -48,7 +48,9 @@ enum {
 	PRT_OPT_COUNT_STATS = 9,  /* 1: count node visits / triangle tests (slower) */
 	PRT_OPT_TIME_KERNEL = 10, /* 1: time the traversal kernel with HIP events (synchronises) */
 	PRT_OPT_ALLOC_LIMIT = 11, /* refuse device allocations above this many bytes (0 = none) */
-	PRT_OPT_MARGIN_REL_LOG2 = 12 /* culling margin relative part 2^-k, k in 4..30 (default 10) */
+	PRT_OPT_MARGIN_REL_LOG2 = 12, /* culling margin relative part 2^-k, k in 4..30 (default 10) */
+	PRT_OPT_BUILDER = 13      /* binary tree under the BVH8: 0 radix tree over the Morton order, 1
+	                             clustering of Morton-order neighbours (default); at set_tris */
 };
 
 /* Statistics (prt_mi355x_get_stat). */

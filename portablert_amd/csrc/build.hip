@@ -42,7 +42,12 @@ enum : int {
 	M_LVL_BEGIN = 3,   // current level [begin, end)
 	M_LVL_END = 4,
 	M_DEPTH = 5,
-	M_SCENE = 8,       // per scene 12 words: ordered centroid min xyz, max xyz, box min, max
+	M_PM = 6,          // clustering: live clusters, and after a step the next count
+	M_PMNEXT = 7,
+	M_PNEXT = 8,       // clustering: internal node allocation counter
+	M_PDONE = 9,       // clustering: finished (a step merged nothing)
+	M_PFINAL = 10,     // clustering: which cluster buffer holds the roots
+	M_SCENE = 12,      // per scene 12 words: ordered centroid min xyz, max xyz, box min, max
 	M_PER_SCENE = 12
 };
 constexpr uint32_t misc_words(uint32_t scenes) { return M_SCENE + M_PER_SCENE * scenes; }
@@ -459,6 +464,159 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	}
 }
 
+// ------------------------------------------------- clustered tree (PLOC), the other builder
+//
+// Bottom-up agglomerative clustering over the key order (Meister and Bittner 2018, "Parallel
+// Locally-Ordered Clustering"): every cluster finds its nearest neighbour among the clusters
+// within kPlocR positions (same scene only) by the surface area of their union; mutual nearest
+// neighbours merge into a new internal node; an ordered compaction removes the merged partners;
+// repeat until a step merges nothing (one cluster per scene). Boxes and counts are computed at
+// the merge, so no bottom-up pass is needed. Node numbering as for the radix tree: internal
+// nodes 0..n-2 (allocation order), leaf of sorted triangle j = n-1+j.
+
+constexpr int kPlocR = 16;
+constexpr int kPlocMaxSteps = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_ploc_init(const float4 *__restrict__ tbox, const uint32_t *__restrict__ sorted,
+                                                      BinTree t, uint32_t *__restrict__ clu, uint32_t *misc) {
+	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	if (j == 0) {
+		misc[M_PM] = t.n;
+		misc[M_PMNEXT] = t.n;
+		misc[M_PNEXT] = 0u;
+		misc[M_PDONE] = 0u;
+		misc[M_PFINAL] = 0u;
+	}
+	if (j >= t.n) return;
+	const uint32_t node = t.n - 1 + j, tri = sorted[j];
+	t.box[2ull * node] = tbox[2ull * tri];  // w: the scene index
+	t.box[2ull * node + 1] = tbox[2ull * tri + 1];
+	t.count[node] = 1u;
+	t.first[node] = j;
+	clu[j] = node;
+}
+
+// Nearest neighbour within +-kPlocR by (area of the union, lower index, higher index): a
+// symmetric order, so a scene's smallest pair is always mutual and every step merges.
+__global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *__restrict__ clu, BinTree t, const uint32_t *misc,
+                                                    int32_t *__restrict__ nn) {
+	__shared__ float4 slo[kBlock + 2 * kPlocR], shi[kBlock + 2 * kPlocR];
+	if (misc[M_PDONE]) return;
+	const int m = static_cast<int>(misc[M_PM]);
+	const int base = static_cast<int>(blockIdx.x) * kBlock - kPlocR;
+	for (int k = threadIdx.x; k < kBlock + 2 * kPlocR; k += kBlock) {
+		const int idx = base + k;
+		if (idx >= 0 && idx < m) {
+			const uint32_t node = clu[idx];
+			slo[k] = t.box[2ull * node];
+			shi[k] = t.box[2ull * node + 1];
+		}
+	}
+	__syncthreads();
+	const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+	if (i >= m) return;
+	const int me = static_cast<int>(threadIdx.x) + kPlocR;
+	const float4 a = slo[me], b = shi[me];
+	float best = __builtin_inff();
+	int bj = -1;
+	for (int d = -kPlocR; d <= kPlocR; ++d) {
+		const int j = i + d;
+		if (d == 0 || j < 0 || j >= m) continue;
+		const float4 c = slo[me + d], e = shi[me + d];
+		if (__float_as_uint(c.w) != __float_as_uint(a.w)) continue;  // other scene
+		const float dx = fmaxf(b.x, e.x) - fminf(a.x, c.x);
+		const float dy = fmaxf(b.y, e.y) - fminf(a.y, c.y);
+		const float dz = fmaxf(b.z, e.z) - fminf(a.z, c.z);
+		const float ar = dx * dy + dx * dz + dy * dz;
+		bool take = ar < best;
+		if (ar == best && bj >= 0) {
+			const int lo = min(i, j), hi = max(i, j), blo = min(i, bj), bhi = max(i, bj);
+			take = lo < blo || (lo == blo && hi < bhi);
+		}
+		if (take) {
+			best = ar;
+			bj = j;
+		}
+	}
+	nn[i] = bj;
+}
+
+// Mutual pairs (i < j) merge at position i; the partner's position is dropped by the compaction.
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restrict__ clu, const int32_t *__restrict__ nn,
+                                                       BinTree t, uint32_t *__restrict__ clu_next,
+                                                       uint32_t *__restrict__ flags, uint32_t n_flags, uint32_t *misc) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n_flags || misc[M_PDONE]) return;
+	const uint32_t m = misc[M_PM];
+	if (i >= m) {
+		flags[i] = 0u;
+		return;
+	}
+	const int32_t j = nn[i];
+	uint32_t keep = 1u, node = clu[i];
+	if (j >= 0 && nn[j] == static_cast<int32_t>(i)) {
+		if (static_cast<int32_t>(i) < j) {
+			const uint32_t id = atomicAdd(misc + M_PNEXT, 1u);
+			if (id < t.n - 1) {
+				const uint32_t l = node, r = clu[j];
+				const float4 la = t.box[2ull * l], lb = t.box[2ull * l + 1];
+				const float4 ra = t.box[2ull * r], rb = t.box[2ull * r + 1];
+				t.box[2ull * id] = make_float4(fminf(la.x, ra.x), fminf(la.y, ra.y), fminf(la.z, ra.z), la.w);
+				t.box[2ull * id + 1] = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z), 0.0f);
+				t.left[id] = l;
+				t.right[id] = r;
+				t.count[id] = t.count[l] + t.count[r];
+				t.first[id] = t.first[l];
+				node = id;
+			} else {
+				atomicOr(misc + M_ERR, 2u);  // more merges than internal nodes: never happens
+			}
+		} else {
+			keep = 0u;
+		}
+	}
+	clu_next[i] = node;
+	flags[i] = keep;
+}
+
+// Ordered compaction (flags scanned in place: exclusive prefix per 4096-entry segment + seg[]).
+__global__ __launch_bounds__(kBlock) void k_ploc_compact(const uint32_t *__restrict__ clu_next,
+                                                         const int32_t *__restrict__ nn, const uint32_t *__restrict__ pre,
+                                                         const uint32_t *__restrict__ seg, uint32_t *__restrict__ clu_out,
+                                                         uint32_t *misc) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (misc[M_PDONE]) return;
+	const uint32_t m = misc[M_PM];
+	if (i >= m) return;
+	const int32_t j = nn[i];
+	const bool keep = !(j >= 0 && nn[j] == static_cast<int32_t>(i) && static_cast<int32_t>(i) > j);
+	const uint32_t pos = pre[i] + seg[i / kScanSeg];
+	if (keep) clu_out[pos] = clu_next[i];
+	if (i == m - 1) misc[M_PMNEXT] = pos + (keep ? 1u : 0u);
+}
+
+// End of a step: the next cluster count; a step that merged nothing ends the clustering, and
+// the buffer it compacted into (parity) holds the roots.
+__global__ void k_ploc_step(uint32_t *misc, uint32_t parity_out) {
+	if (threadIdx.x != 0 || misc[M_PDONE]) return;
+	const uint32_t m = misc[M_PM], mn = misc[M_PMNEXT];
+	misc[M_PFINAL] = parity_out;
+	misc[M_PM] = mn;
+	if (mn == m) misc[M_PDONE] = 1u;
+}
+
+// One cluster per non-empty scene remains: it is the scene's root.
+__global__ __launch_bounds__(kBlock) void k_ploc_roots(const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
+                                                       BinTree t, uint32_t *__restrict__ scene_root, uint32_t scenes,
+                                                       uint32_t *misc) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i == 0 && !misc[M_PDONE]) atomicOr(misc + M_ERR, 2u);  // clustering did not finish
+	if (i >= misc[M_PM]) return;
+	const uint32_t node = (misc[M_PFINAL] ? c1 : c0)[i];
+	const uint32_t sc = __float_as_uint(t.box[2ull * node].w);
+	if (sc >= scenes || atomicExch(scene_root + sc, node) != kNone) atomicOr(misc + M_ERR, 2u);
+}
+
 // ------------------------------------------------------------------------- BVH8 collapse
 
 __device__ __forceinline__ float half_area(float4 lo, float4 hi) {
@@ -619,8 +777,22 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 					++inner;
 				} else {
 					meta = static_cast<uint8_t>((cnt[k] << 5) | toff);
-					const uint32_t first = t.first[el[k]];
-					for (uint32_t m = 0; m < cnt[k]; ++m) a.tri_dest[first + m] = tri_base + toff + m;
+					// the subtree's triangles in depth-first (left to right) order: a range of
+					// sorted positions for the radix tree, any set of them for the clustered tree
+					uint32_t stk[kMaxLeaf + 1];
+					int sp = 0;
+					stk[sp++] = el[k];
+					uint32_t m = 0;
+					while (sp > 0 && m < cnt[k]) {
+						const uint32_t x = stk[--sp];
+						if (x >= n_int) {
+							a.tri_dest[x - n_int] = tri_base + toff + m;
+							++m;
+						} else if (sp + 2 <= kMaxLeaf + 1) {
+							stk[sp++] = t.right[x];
+							stk[sp++] = t.left[x];
+						}
+					}
 					toff += cnt[k];
 				}
 			}
@@ -732,12 +904,6 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	const uint32_t words = misc_words(S);
 	const uint32_t *off = S > 1 ? ss.tri_off : nullptr;
 
-	hipEvent_t ev0 = nullptr, ev1 = nullptr;
-	if (build_ms) {
-		PRT_TRY(hipEventCreate(&ev0));
-		PRT_TRY(hipEventCreate(&ev1));
-		PRT_TRY(hipEventRecord(ev0, st));
-	}
 	PRT_TRY(s.tbox.ensure(sizeof(float4) * 2 * n));
 	for (int k = 0; k < 2; ++k) {
 		PRT_TRY(s.keys[k].ensure(sizeof(uint64_t) * n));
@@ -757,6 +923,12 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	PRT_TRY(out.nodes_f.ensure(sizeof(NodeF) * cap));
 	PRT_TRY(out.tris.ensure(sizeof(TriRec) * n));
 
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	if (build_ms) {  // device time of the build kernels (allocations above are not counted)
+		PRT_TRY(hipEventCreate(&ev0));
+		PRT_TRY(hipEventCreate(&ev1));
+		PRT_TRY(hipEventRecord(ev0, st));
+	}
 	uint32_t *misc = s.misc.as<uint32_t>();
 	const float *tri_in = ss.tris ? ss.tris : s.tri_in.as<float>();
 	hipLaunchKernelGGL(k_init_misc, dim3(blocks_for(words)), dim3(kBlock), 0, st, misc, words);
@@ -790,11 +962,42 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	t.box = s.bbox.as<float4>();
 	t.n = n;
 	uint32_t *scene_root = s.scene_root.as<uint32_t>();
-	hipLaunchKernelGGL(k_scene_init, dim3(blocks_for(S)), dim3(kBlock), 0, st, off, S, n, scene_root);
-	if (n > 1)
-		hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, keys, t, scene_root);
-	hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
-	                   s.vals[cur].as<uint32_t>(), t);
+	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
+	if (p.builder == 1 && !fixed) {
+		// clustering: steps until one merges nothing, the host checks every 8 steps
+		const uint32_t n_seg_p = blocks_for(n, kScanSeg);
+		PRT_TRY(s.ploc.ensure(sizeof(uint32_t) * (5ull * n + n_seg_p + 8)));
+		uint32_t *pw = s.ploc.as<uint32_t>();
+		uint32_t *clu0 = pw, *clu1 = pw + n, *flags = pw + 3ull * n, *clu_next = pw + 4ull * n, *segp = pw + 5ull * n;
+		int32_t *nnb = reinterpret_cast<int32_t *>(pw + 2ull * n);
+		hipLaunchKernelGGL(k_fill, dim3(blocks_for(S)), dim3(kBlock), 0, st, scene_root, S, kNone);
+		hipLaunchKernelGGL(k_ploc_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
+		                   s.vals[cur].as<uint32_t>(), t, clu0, misc);
+		bool done = false;
+		for (int it = 0; it < kPlocMaxSteps && !done; ++it) {
+			uint32_t *cin = (it & 1) ? clu1 : clu0, *cout = (it & 1) ? clu0 : clu1;
+			hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(n)), dim3(kBlock), 0, st, cin, t, misc, nnb);
+			hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(n)), dim3(kBlock), 0, st, cin, nnb, t, clu_next, flags, n,
+			                   misc);
+			hipLaunchKernelGGL(k_scan_seg, dim3(n_seg_p), dim3(kBlock), 0, st, flags, n, segp);
+			hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, segp, n_seg_p);
+			hipLaunchKernelGGL(k_ploc_compact, dim3(blocks_for(n)), dim3(kBlock), 0, st, clu_next, nnb, flags, segp, cout,
+			                   misc);
+			hipLaunchKernelGGL(k_ploc_step, dim3(1), dim3(64), 0, st, misc, (it & 1) ? 0u : 1u);
+			if ((it & 7) == 7) {
+				PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
+				PRT_TRY(hipStreamSynchronize(st));
+				done = hm[M_PDONE] != 0u;
+			}
+		}
+		hipLaunchKernelGGL(k_ploc_roots, dim3(blocks_for(n)), dim3(kBlock), 0, st, clu0, clu1, t, scene_root, S, misc);
+	} else {
+		hipLaunchKernelGGL(k_scene_init, dim3(blocks_for(S)), dim3(kBlock), 0, st, off, S, n, scene_root);
+		if (n > 1)
+			hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, keys, t, scene_root);
+		hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
+		                   s.vals[cur].as<uint32_t>(), t);
+	}
 	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
 	hipLaunchKernelGGL(k_roots, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, s.bin_of.as<uint32_t>(), scene_root, S);
 	CollapseArgs ca;
@@ -808,7 +1011,6 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	ca.param = ss.param;
 	ca.leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size));
 	ca.width = std::max(2, std::min(8, p.width));
-	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
 	const int max_levels = fixed ? ss.levels : 128;
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
 	for (int level = 0; level < max_levels; ++level) {
@@ -843,6 +1045,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		(void)hipEventDestroy(ev1);
 	}
 	*bad_input = (hm[M_ERR] & 1u) != 0;
+	if (*bad_input) return hipSuccess;  // the caller reports the input; the tree is not used
 	if (hm[M_ERR] & 2u) return hipErrorUnknown;  // inconsistent build
 	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SCENE + 6 + k]);
 	out.n_nodes = hm[M_NODES];

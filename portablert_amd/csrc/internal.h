@@ -63,12 +63,15 @@ struct BuildScratch {
 	DevBuf tri_dest;    // sorted position -> TriRec slot
 	DevBuf misc;        // counters and per-scene bounds
 	DevBuf scene_root;  // binary root node of every scene
+	DevBuf ploc;        // clustering builder: cluster lists, neighbours, flags
 	void *host_misc = nullptr;  // pinned mirror of the counters and scene 0's bounds
 };
 
 struct BuildParams {
 	int leaf_size = 2;  // 1..kMaxLeaf
 	int width = 8;      // children per BVH8 node, 2..8 (the node format has 8 slots)
+	int builder = 1;    // binary tree: 0 radix tree over the Morton order (Karras), 1 clustering
+	                    // (PLOC) over the same order; graph-capturable builds use the radix tree
 };
 
 constexpr uint32_t kMaxScenes = 65536;
