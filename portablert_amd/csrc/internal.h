@@ -68,7 +68,7 @@ struct BuildScratch {
 };
 
 struct BuildParams {
-	int leaf_size = 2;  // 1..kMaxLeaf
+	int leaf_size = 3;  // 1..kMaxLeaf
 	int width = 8;      // children per BVH8 node, 2..8 (the node format has 8 slots)
 	int builder = 1;    // binary tree: 0 radix tree over the Morton order (Karras), 1 clustering
 	                    // (PLOC) over the same order; graph-capturable builds use the radix tree
@@ -108,8 +108,8 @@ struct TraceParams {
 	int block = 128;          // threads per workgroup
 	int waves_per_cu = 24;    // persistent waves per CU
 	int lds_stack = 16;       // LDS stack entries per thread (power of two)
-	int chunk = 64;           // rays a wave reserves per atomic (refilled lane by lane)
-	int leaf_phase = 16;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
+	int chunk = 128;          // rays a wave reserves per atomic (refilled lane by lane)
+	int leaf_phase = 24;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int count_stats = 0;      // count node visits and triangle tests
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)

@@ -76,7 +76,7 @@ def test_edge_fixtures(gpu_ctx, name):
         want = oracle.trace(tris, rays, oracle.LOWEST_ID)
         assert_same(got, want, what=f"{name} leaf={leaf} vs oracle")
         assert_same(got, ref, ids=False, what=f"{name} vs reference")
-    gpu_ctx.set_option("leaf_size", 2)
+    gpu_ctx.set_option("leaf_size", 3)  # back to the default
 
 
 @pytest.mark.parametrize("name", ["workload0", "workload1_primary", "workload1_bounce", "workload2"])
