@@ -49,8 +49,9 @@ enum {
 	PRT_OPT_TIME_KERNEL = 10, /* 1: time the traversal kernel with HIP events (synchronises) */
 	PRT_OPT_ALLOC_LIMIT = 11, /* refuse device allocations above this many bytes (0 = none) */
 	PRT_OPT_MARGIN_REL_LOG2 = 12, /* culling margin relative part 2^-k, k in 4..30 (default 10) */
-	PRT_OPT_BUILDER = 13      /* binary tree under the BVH8: 0 radix tree over the Morton order, 1
+	PRT_OPT_BUILDER = 13,     /* binary tree under the BVH8: 0 radix tree over the Morton order, 1
 	                             clustering of Morton-order neighbours (default); at set_tris */
+	PRT_OPT_REFILL = 14       /* a wave loads new rays once at least v of its lanes are idle, 1..64 (default 8) */
 };
 
 /* Statistics (prt_mi355x_get_stat). */

@@ -288,6 +288,10 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		if (v < 1 || v > 64) return bad();
 		c->tp.leaf_phase = static_cast<int>(v);
 		break;
+	case PRT_OPT_REFILL:
+		if (v < 1 || v > 64) return bad();
+		c->tp.refill = static_cast<int>(v);
+		break;
 	case PRT_OPT_COUNT_STATS: c->tp.count_stats = v ? 1 : 0; break;
 	case PRT_OPT_TIME_KERNEL: c->time_kernel = v ? 1 : 0; break;
 	case PRT_OPT_ALLOC_LIMIT:
@@ -319,6 +323,7 @@ int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
 	case PRT_OPT_LDS_STACK: *v = c->tp.lds_stack; break;
 	case PRT_OPT_CHUNK: *v = c->tp.chunk; break;
 	case PRT_OPT_LEAF_PHASE: *v = c->tp.leaf_phase; break;
+	case PRT_OPT_REFILL: *v = c->tp.refill; break;
 	case PRT_OPT_COUNT_STATS: *v = c->tp.count_stats; break;
 	case PRT_OPT_TIME_KERNEL: *v = c->time_kernel; break;
 	case PRT_OPT_ALLOC_LIMIT: *v = c->alloc_limit; break;

@@ -763,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 		uint32_t inner = 0, toff = 0;
 #pragma unroll
 		for (int k = 0; k < 8; ++k) {
-			uint8_t meta = 0;
+			uint8_t meta = kMetaEmpty;
 			float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 			float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 			if (k < ne) {
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 					a.bin_of[child_base + inner] = el[k];
 					++inner;
 				} else {
-					meta = static_cast<uint8_t>((cnt[k] << 5) | toff);
+					meta = meta_leaf(toff, cnt[k]);
 					// the subtree's triangles in depth-first (left to right) order: a range of
 					// sorted positions for the radix tree, any set of them for the clustered tree
 					uint32_t stk[kMaxLeaf + 1];

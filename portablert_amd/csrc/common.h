@@ -20,13 +20,19 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "TriRec");
 
-// meta byte of a BVH8 child slot: 0 empty; internal child: 0xE0 | offset from child_base;
-// leaf child: (count << 5) | offset from tri_base, count in 1..kMaxLeaf.
+// meta byte of a BVH8 child slot: 0xFF empty; internal child: 0x80 | offset from child_base
+// (offset < 8); leaf child: (offset from tri_base) << 2 | (count - 1), count in 1..kMaxLeaf,
+// offset < 32. A slot's stack entry is then one select and one add (child_entry).
 constexpr int kMaxLeaf = 4;
-constexpr uint8_t kMetaInternal = 0xE0;
-__host__ __device__ inline bool meta_internal(uint32_t m) { return (m & 0xE0u) == 0xE0u; }
-__host__ __device__ inline uint32_t meta_count(uint32_t m) { return m >> 5; }
-__host__ __device__ inline uint32_t meta_offset(uint32_t m) { return m & 0x1Fu; }
+constexpr uint8_t kMetaEmpty = 0xFF;
+constexpr uint8_t kMetaInternal = 0x80;
+__host__ __device__ inline bool meta_used(uint32_t m) { return m != kMetaEmpty; }
+__host__ __device__ inline bool meta_internal(uint32_t m) { return m != kMetaEmpty && (m & 0x80u) != 0u; }
+__host__ __device__ inline uint32_t meta_count(uint32_t m) { return (m & 3u) + 1u; }
+__host__ __device__ inline uint32_t meta_offset(uint32_t m) { return (m & 0x80u) ? (m & 0x7Fu) : (m >> 2); }
+__host__ __device__ inline uint8_t meta_leaf(uint32_t offset, uint32_t count) {
+	return static_cast<uint8_t>((offset << 2) | (count - 1u));
+}
 
 // 8-bit quantized BVH8 node, 80 bytes. Child k's box on axis a is
 // [p_a + qlo_a[k] * 2^e_a, p_a + qhi_a[k] * 2^e_a], evaluated in fp32 as fmaf(q, 2^e, p);
@@ -53,12 +59,21 @@ struct alignas(16) NodeF {
 };
 static_assert(sizeof(NodeF) == 208, "NodeF");
 
-// Stack / work entry: bit 31 set = leaf (bits 29..30 = count-1, bits 0..28 = first
-// triangle record); clear = internal node index.
+// Stack / work entry: bit 31 set = leaf (bits 2..30 = first triangle record, bits 0..1 =
+// count-1); clear = internal node index. Adding 3 to a leaf entry advances it by one triangle
+// with one fewer left.
 constexpr uint32_t kLeafBit = 0x80000000u;
-constexpr uint32_t kTriMask = 0x1FFFFFFFu;
+constexpr uint32_t kTriMask = 0x1FFFFFFFu;  // largest first-triangle index
 __host__ __device__ inline uint32_t leaf_entry(uint32_t first, uint32_t count) {
-	return kLeafBit | ((count - 1u) << 29) | first;
+	return kLeafBit | (first << 2) | (count - 1u);
+}
+__host__ __device__ inline uint32_t leaf_first(uint32_t e) { return (e & ~kLeafBit) >> 2; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t e) { return (e & 3u) + 1u; }
+// Entry base for a node's leaf children: leaf_base(tri_base) + meta byte = the leaf's entry.
+__host__ __device__ inline uint32_t leaf_base(uint32_t tri_base) { return kLeafBit | (tri_base << 2); }
+// Stack entry of a used child slot with meta byte m.
+__host__ __device__ inline uint32_t child_entry(uint32_t m, uint32_t child_base, uint32_t lbase) {
+	return ((m & 0x80u) ? child_base : lbase) + (m & 0x7Fu);
 }
 
 // Scale 2^e as a normal float; e is kept in [-126, 127] by the builder.

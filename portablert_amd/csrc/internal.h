@@ -110,6 +110,7 @@ struct TraceParams {
 	int lds_stack = 16;       // LDS stack entries per thread (power of two)
 	int chunk = 128;          // rays a wave reserves per atomic (refilled lane by lane)
 	int leaf_phase = 24;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
+	int refill = 8;           // load new rays once at least this many lanes of a wave are idle
 	int count_stats = 0;      // count node visits and triangle tests
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)

@@ -77,7 +77,7 @@ hipError_t build_clusters(prt_ctx *c) {
 		uint64_t s = 0;
 		for (int k = 0; k < 8; ++k) {
 			const uint32_t m = nodes[i].meta[k];
-			if (!m) continue;
+			if (!meta_used(m)) continue;
 			s += meta_internal(m) ? cnt[nodes[i].child_base + meta_offset(m)] : meta_count(m);
 		}
 		cnt[i] = s;
@@ -97,7 +97,7 @@ hipError_t build_clusters(prt_ctx *c) {
 			root.box[3 + a] = -__builtin_inff();
 		}
 		for (int k = 0; k < 8; ++k)
-			if (nodes[0].meta[k])
+			if (meta_used(nodes[0].meta[k]))
 				for (int a = 0; a < 3; ++a) {
 					root.box[a] = std::min(root.box[a], nodes[0].lo[a][k]);
 					root.box[3 + a] = std::max(root.box[3 + a], nodes[0].hi[a][k]);
@@ -108,12 +108,12 @@ hipError_t build_clusters(prt_ctx *c) {
 		const Item it = open.top();
 		const NodeF &n = nodes[it.entry];
 		int kids = 0;
-		for (int k = 0; k < 8; ++k) kids += n.meta[k] != 0;
+		for (int k = 0; k < 8; ++k) kids += meta_used(n.meta[k]);
 		if (open.size() + done.size() - 1 + kids > static_cast<size_t>(kClusters)) break;
 		open.pop();
 		for (int k = 0; k < 8; ++k) {
 			const uint32_t m = n.meta[k];
-			if (!m) continue;
+			if (!meta_used(m)) continue;
 			Item ch{};
 			for (int a = 0; a < 3; ++a) {
 				ch.box[a] = n.lo[a][k];
@@ -147,13 +147,13 @@ hipError_t build_clusters(prt_ctx *c) {
 			const uint32_t e = stack.back();
 			stack.pop_back();
 			if (e & kLeafBit) {
-				const uint32_t first = e & kTriMask, count = ((e >> 29) & 3u) + 1u;
+				const uint32_t first = leaf_first(e), count = leaf_count(e);
 				for (uint32_t q = 0; q < count; ++q) tri_cluster[tris[first + q].id] = static_cast<uint32_t>(ci);
 			} else {
 				const NodeF &n = nodes[e];
 				for (int k = 0; k < 8; ++k) {
 					const uint32_t m = n.meta[k];
-					if (!m) continue;
+					if (!meta_used(m)) continue;
 					stack.push_back(meta_internal(m) ? n.child_base + meta_offset(m)
 					                                 : leaf_entry(n.tri_base + meta_offset(m), meta_count(m)));
 				}

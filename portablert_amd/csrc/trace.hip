@@ -40,6 +40,7 @@ struct KArgs {
 	uint32_t spill_depth;
 	uint32_t chunk;         // rays a wave reserves per atomic
 	uint32_t leaf_phase;    // triangle phase once parked lanes >= live lanes * leaf_phase / 64
+	uint32_t refill;        // load new rays once at least this many lanes are idle (or all are)
 	float margin_rel;
 	float margin_abs;
 	int cull;
@@ -87,8 +88,10 @@ __device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float 
 	const float big = 0x1p60f;
 	r.fast = fast_scene && !r.degen && !r.patho && fabsf(r.inv[0]) < big && fabsf(r.inv[1]) < big &&
 	         fabsf(r.inv[2]) < big && fabsf(r.o[0]) < big && fabsf(r.o[1]) < big && fabsf(r.o[2]) < big;
-	const float len = sqrtf(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
-	r.abs_margin = len > 0.0f ? margin_abs / len : __builtin_inff();
+	// margin_abs / |d|, rounded up (1 + 2^-20 covers the reciprocal square root's error); no
+	// absolute margin limit for tiny |d|
+	const float len2 = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
+	r.abs_margin = len2 >= 0x1p-100f ? margin_abs * (__builtin_amdgcn_rsqf(len2) * (1.0f + 0x1p-20f)) : __builtin_inff();
 	r.tb = __builtin_inff();
 	r.ub = r.vb = 0.0f;
 	r.idb = kInvalid;
@@ -135,16 +138,17 @@ __device__ __forceinline__ float child_slab(const Ray &r, const float lo[3], con
 }
 
 // LDS ring stack: the newest `S` entries of each lane stay in LDS, older ones are moved to
-// the lane's spill region (entry at depth k lives at spill[k] once evicted).
+// the lane's spill region (entry at depth k lives at spill[k] once evicted). An entry is
+// {work entry, sort key of its box entry distance}.
 template <int S, int B>
 struct Stack {
 	uint2 *lds;    // [S][B]
 	uint2 *spill;  // this lane's spill region
 	uint32_t sp;
-	__device__ __forceinline__ void push(uint32_t ref, float tmin) {
+	__device__ __forceinline__ void push(uint32_t ref, uint32_t key) {
 		uint2 *slot = lds + (sp & (S - 1)) * B;
 		if (sp >= static_cast<uint32_t>(S)) spill[sp - S] = *slot;
-		*slot = make_uint2(ref, __float_as_uint(tmin));
+		*slot = make_uint2(ref, key);
 		++sp;
 	}
 	__device__ __forceinline__ uint2 pop() {
@@ -158,8 +162,8 @@ struct Stack {
 
 // Sort key of a child hit: the order-preserving integer image of its entry distance with the
 // low three bits replaced by the child slot. Clearing bits only lowers the distance the key
-// decodes to, so re-culling with the decoded value stays conservative. -inf (a box that must
-// pass) is clamped to -FLT_MAX, which every culling limit admits. Misses sort last.
+// stands for, so re-culling by key at pop stays conservative. -inf (a box that must pass) is
+// clamped to -FLT_MAX, which every culling limit admits. Misses sort last.
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t sort_key(float tmin, int slot) {
@@ -168,9 +172,11 @@ __device__ __forceinline__ uint32_t sort_key(float tmin, int slot) {
 	return (o & ~7u) | static_cast<uint32_t>(slot);
 }
 
-__device__ __forceinline__ float key_tmin(uint32_t key) {
-	const uint32_t o = key & ~7u;
-	return __uint_as_float(o ^ (static_cast<uint32_t>(~static_cast<int32_t>(o) >> 31) | 0x80000000u));
+// Largest key whose decoded distance is within `limit` (not NaN): a key k passes iff
+// k <= key_limit(limit). The decoded distance of k is at most the box's entry distance.
+__device__ __forceinline__ uint32_t key_limit(float limit) {
+	const uint32_t u = __float_as_uint(limit);
+	return (u ^ (static_cast<uint32_t>(static_cast<int32_t>(u) >> 31) | 0x80000000u)) | 7u;
 }
 
 #define PRT_CE(i, j)                                                                               \
@@ -192,12 +198,11 @@ __device__ __forceinline__ void sort8(uint32_t k[8]) {
 // to turn a key back into a stack entry.
 struct NodeHits {
 	uint32_t key[8];
-	uint32_t child_base, tri_base, meta_lo, meta_hi;
+	uint32_t child_base, lbase;
+	uint64_t meta;  // the 8 meta bytes, slot k in bits 8k..8k+7
 	__device__ __forceinline__ uint32_t entry(uint32_t key) const {
-		const uint32_t slot = key & 7u;
-		const uint32_t meta = ((slot < 4u ? meta_lo : meta_hi) >> (8u * (slot & 3u))) & 0xFFu;
-		return meta_internal(meta) ? child_base + meta_offset(meta)
-		                           : leaf_entry(tri_base + meta_offset(meta), meta_count(meta));
+		const uint32_t m = static_cast<uint32_t>(meta >> ((key << 3) & 63u));
+		return child_entry(m & 0xFFu, child_base, lbase);
 	}
 };
 
@@ -220,10 +225,9 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 	const float s[3] = {exp2i(static_cast<int8_t>(w0.w & 0xFF)), exp2i(static_cast<int8_t>((w0.w >> 8) & 0xFF)),
 	                    exp2i(static_cast<int8_t>((w0.w >> 16) & 0xFF))};
 	h.child_base = w1.x;
-	h.tri_base = w1.y;
+	h.lbase = leaf_base(w1.y);
 	const uint32_t meta_lo = w1.z, meta_hi = w1.w;
-	h.meta_lo = meta_lo;
-	h.meta_hi = meta_hi;
+	h.meta = static_cast<uint64_t>(meta_lo) | (static_cast<uint64_t>(meta_hi) << 32);
 	// qlo x: w2.x w2.y, qlo y: w2.z w2.w, qlo z: w3.x w3.y, qhi x: w3.z w3.w, qhi y: w4.x w4.y, qhi z: w4.z w4.w
 	const uint32_t ql[3][2] = {{w2.x, w2.y}, {w2.z, w2.w}, {w3.x, w3.y}};
 	const uint32_t qh[3][2] = {{w3.z, w3.w}, {w4.x, w4.y}, {w4.z, w4.w}};
@@ -264,7 +268,7 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			}
 			const float tmin = fmaxf(fmaxf(tn[0], tn[1]), tn[2]);
 			const float tmax = fminf(fminf(tf[0], tf[1]), tf[2]);
-			const bool hit = meta != 0u && tmax >= 0.0f && tmin <= fminf(tmax, limit);
+			const bool hit = meta != kMetaEmpty && tmax >= 0.0f && tmin <= fminf(tmax, limit);
 			h.key[k] = hit ? sort_key(tmin, k) : kMiss;
 		}
 		sort8(h.key);
@@ -283,7 +287,7 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			hi[ax] = fmaf(static_cast<float>(bh), s[ax], p[ax]);
 		}
 		const float tmin = child_slab(r, lo, hi, limit);
-		const bool hit = meta != 0u && !__builtin_isnan(tmin);
+		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		h.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
 	sort8(h.key);
@@ -295,9 +299,8 @@ __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_
 	const NodeF *n = nodes + idx;
 	const uint4 h = *reinterpret_cast<const uint4 *>(n);
 	hh.child_base = h.x;
-	hh.tri_base = h.y;
-	hh.meta_lo = h.z;
-	hh.meta_hi = h.w;
+	hh.lbase = leaf_base(h.y);
+	hh.meta = static_cast<uint64_t>(h.z) | (static_cast<uint64_t>(h.w) << 32);
 	float lo[3][8], hi[3][8];
 #pragma unroll
 	for (int ax = 0; ax < 3; ++ax) {
@@ -315,15 +318,25 @@ __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_
 		const float l[3] = {lo[0][k], lo[1][k], lo[2][k]};
 		const float u[3] = {hi[0][k], hi[1][k], hi[2][k]};
 		const float tmin = child_slab(r, l, u, limit);
-		const bool hit = meta != 0u && !__builtin_isnan(tmin);
+		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		hh.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
 	sort8(hh.key);
 }
 
+// Writes the finished ray's record in the caller's layout.
+__device__ __forceinline__ void write_record(const KArgs &a, uint64_t ray_idx, const Ray &r) {
+	Hit h;
+	h.t = r.tb;
+	h.u = r.ub;
+	h.v = r.vb;
+	h.id = r.idb;
+	write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
+}
+
 __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ray &r) {
-	const uint32_t first = entry & kTriMask;
-	const uint32_t count = ((entry >> 29) & 3u) + 1u;
+	const uint32_t first = leaf_first(entry);
+	const uint32_t count = leaf_count(entry);
 	for (uint32_t m = 0; m < count; ++m) {
 		const uint4 *src = reinterpret_cast<const uint4 *>(tris + first + m);
 		const uint4 a = src[0], b = src[1], c = src[2];
@@ -370,9 +383,11 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0, c_nphase = 0, c_tphase = 0, c_culled = 0;
 
 	while (true) {
-		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle)
+		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle); the
+		// ray loads are amortised over at least `refill` lanes unless the whole wave is idle
 		const uint64_t need = __ballot(!active);
-		if (need != 0ull && !exhausted) {
+		const uint32_t n_need = __popcll(need);
+		if (n_need != 0u && !exhausted && (n_need >= a.refill || n_need == __popcll(__ballot(true)))) {
 			if (chunk_next >= chunk_end && BATCH) {
 				uint64_t tix = 0;
 				if (lane == 0) tix = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter), 1ull);
@@ -415,7 +430,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 					// (best on top): their subtrees are intersected exactly first and seed the
 					// culling distance; the root traversal afterwards covers every triangle
 					// except those of the clusters already searched
-					st.push(0u, -3.402823466e38f);
+					st.push(0u, sort_key(-3.402823466e38f, 0));
 					cur = kInvalid;
 					root_phase = false;
 					seen = 0u;
@@ -430,13 +445,13 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 							const float tmin = child_slab(r, lo, hi, __builtin_inff());
 							if (!__builtin_isnan(tmin)) {
 								sd[q] = a.cl_entry[sc[q]];
-								st.push(sd[q], fmaxf(tmin, -3.402823466e38f));
+								st.push(sd[q], sort_key(tmin, 0));
 							}
 						}
 					}
 				}
 			}
-			const uint64_t took = __popcll(need) < avail ? __popcll(need) : avail;
+			const uint64_t took = n_need < avail ? n_need : avail;
 			chunk_next += took;
 		}
 		if (__ballot(active) == 0ull) {
@@ -449,10 +464,10 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 
 		// ---- fetch work: pop one entry; park a leaf if the parking slot is free
 		if (cur == kInvalid && st.sp > 0) {
-			const float limit = cull_limit(r, a);
+			const uint32_t klim = key_limit(cull_limit(r, a));
 			do {
 				const uint2 e = st.pop();
-				if (__uint_as_float(e.y) <= limit) {
+				if (e.y <= klim) {
 					cur = e.x;
 					if (SEEDED && !root_phase) {
 						// a cluster popped here is searched completely before anything below it
@@ -470,12 +485,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 			cur = kInvalid;
 		}
 		if (cur == kInvalid && pend == kInvalid && st.sp == 0) {  // ray finished
-			Hit h;
-			h.t = r.tb;
-			h.u = r.ub;
-			h.v = r.vb;
-			h.id = r.idb;
-			write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
+			write_record(a, ray_idx, r);
 			active = false;
 			continue;
 		}
@@ -493,8 +503,8 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 		}
 		if (tri_phase) {
 			if (pend != kInvalid) {
-				const uint32_t tri = pend & kTriMask;
-				const uint32_t left = ((pend >> 29) & 3u) + 1u;
+				const uint32_t tri = leaf_first(pend);
+				const uint32_t left = leaf_count(pend);
 				if (tri + left > a.n_tris) {  // corrupt entry: report, never read
 					atomicOr(a.counter + 8, 1u);
 					pend = kInvalid;
@@ -502,7 +512,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				}
 				test_leaf(a.tris, leaf_entry(tri, 1u), r);
 				if (COUNT) ++c_tris;
-				pend = left == 1u ? kInvalid : pend + 1u - (1u << 29);  // next triangle, one fewer left
+				pend = left == 1u ? kInvalid : pend + 3u;  // next triangle, one fewer left
 			}
 		} else if (cur != kInvalid && !(cur & kLeafBit)) {
 			if (cur >= a.n_nodes || st.sp + 7u > a.max_entries) {  // corrupt node or stack bound
@@ -529,7 +539,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				if (h.key[k] != kMiss) {
 					const uint32_t e = h.entry(h.key[k]);
 					if (COUNT) c_leaves += (e & kLeafBit) ? 1u : 0u;
-					if (!searched(e)) st.push(e, key_tmin(h.key[k]));
+					if (!searched(e)) st.push(e, h.key[k]);
 				}
 			}
 		}
@@ -627,6 +637,7 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.spill_depth = spill_depth;
 	a.chunk = static_cast<uint32_t>(std::max(64, std::min(1 << 16, p.chunk)));
 	a.leaf_phase = static_cast<uint32_t>(std::max(1, std::min(64, p.leaf_phase)));
+	a.refill = static_cast<uint32_t>(std::max(1, std::min(64, p.refill)));
 	a.margin_rel = p.margin_rel;
 	a.margin_abs = p.margin_abs;
 	a.cull = p.cull;

@@ -118,6 +118,32 @@ def test_workload2_full(gpu_ctx):
     print(f"hit rate {float((got['t'] < np.inf).mean()):.4f} negative-t rate {neg:.4f}", flush=True)
 
 
+@pytest.mark.parametrize("opts", [
+    dict(block=64, lds_stack=8, waves_per_cu=8),
+    dict(block=256, lds_stack=16, waves_per_cu=32),
+    dict(chunk=64, leaf_phase=1, refill=1),
+    dict(chunk=4096, leaf_phase=64, refill=64),
+    dict(refill=33, leaf_phase=40, lds_stack=8),
+])
+def test_scheduling_options(gpu_ctx, opts):
+    """Launch shape, stack size, ray chunks, triangle-phase and refill thresholds change only
+    the schedule: records equal the oracle's for each setting (8-entry stacks spill)."""
+    tris = scenes.scene(2)
+    rays = scenes.rays(2, first=1000000, count=300001)
+    mask = prt.T | prt.PRIMITIVE_ID | prt.UV
+    gpu_ctx.set_tris(tris)
+    saved = {k: gpu_ctx.get_option(k) for k in opts}
+    try:
+        for k, v in opts.items():
+            gpu_ctx.set_option(k, v)
+        got = gpu_ctx.nearest_hits(rays, mask)
+    finally:
+        for k, v in saved.items():
+            gpu_ctx.set_option(k, v)
+    want = oracle.trace(tris, rays, oracle.LOWEST_ID, min(16, os.cpu_count() or 1))
+    assert_same(got, want, fields=("t", "primitive_id", "u", "v"), what=str(opts))
+
+
 def test_all_field_masks(gpu_ctx):
     """Every HitReg<Tags...> layout holds the same values as the full record."""
     tris = scenes.scene(0)
@@ -225,7 +251,7 @@ def test_bvh_invariants(gpu_ctx):
     scale = (2.0 ** e).astype(np.float32)
     qlo = q[:, 32:56].reshape(n_nodes, 3, 8).astype(np.float32)
     qhi = q[:, 56:80].reshape(n_nodes, 3, 8).astype(np.float32)
-    used = meta != 0
+    used = meta != 0xFF
     # fp32 model of fmaf(q, 2^e, p): the product is exact, so one rounding of the sum
     dlo = (qlo * scale[:, :, None]).astype(np.float64) + p[:, :, None].astype(np.float64)
     dhi = (qhi * scale[:, :, None]).astype(np.float64) + p[:, :, None].astype(np.float64)
@@ -235,9 +261,9 @@ def test_bvh_invariants(gpu_ctx):
     # internal children: the parent's fp32 box of the child equals the union of the child
     # node's own child boxes (both come from the same binary-tree boxes)
     cbase = f[:, 0:4].copy().view(np.uint32)[:, 0]
-    inner = (meta & 0xE0) == 0xE0
+    inner = used & ((meta & 0x80) != 0)
     node, slot = np.nonzero(inner)
-    child = cbase[node] + (meta[node, slot] & 0x1F)
+    child = cbase[node] + (meta[node, slot] & 0x7F)
     assert (child < n_nodes).all()
     cu = used[child]
     clo = np.where(cu[:, None, :], lo[child], np.inf).min(axis=2)
@@ -248,8 +274,8 @@ def test_bvh_invariants(gpu_ctx):
     v = tr[:, :9].view(np.float32).reshape(-1, 3, 3)
     for k in range(8):
         m = meta[:, k]
-        leaf = (m != 0) & ((m & 0xE0) != 0xE0)
-        cnt, off = (m >> 5).astype(np.int64), (m & 0x1F).astype(np.int64)
+        leaf = (m & 0x80) == 0
+        cnt, off = (m & 3).astype(np.int64) + 1, (m >> 2).astype(np.int64)
         nodes = np.flatnonzero(leaf)
         first = tbase[nodes].astype(np.int64) + off[nodes]
         for j in range(4):  # leaves hold 1..4 triangles
