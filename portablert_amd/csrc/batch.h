@@ -29,7 +29,7 @@ struct Batch {
 	std::vector<uint64_t> tri_off, ray_off;  // host prefix offsets, n_scenes + 1 each
 	uint32_t mask = 0;
 	BucketParams bucket[kBatchBuckets];
-	int block = 128, waves_per_cu = 24, lds_stack = 16;  // launch shape of the one traversal launch
+	int block = 128, waves_per_cu = 32, lds_stack = 8;  // launch shape of the one traversal launch
 	DevBuf d_tri_off;  // uint32 prefix offsets
 	DevBuf d_param;    // uint32 per scene: leaf | width << 8
 	DevBuf d_box;      // float[6] per scene, written by the build

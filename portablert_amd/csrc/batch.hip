@@ -475,7 +475,7 @@ int prt_mi355x_batch_tune(prt_ctx *c, const float *d_tris9, const float *d_rays6
 	if (e == hipSuccess) e = prt::upload_all_tiles(B, st);
 	double best_shape = 1e30;
 	int shape[3] = {B.block, B.waves_per_cu, B.lds_stack};
-	const int blocks[] = {64, 128, 256}, stacks[] = {8, 16}, waves[] = {16, 24, 32};
+	const int blocks[] = {64, 128, 256}, stacks[] = {8, 16}, waves[] = {20, 24, 32};
 	for (int bl : blocks)
 		for (int sk : stacks)
 			for (int wv : waves) {

@@ -106,8 +106,8 @@ struct TraceParams {
 	int node_format = 1;      // 0 fp32 nodes, 1 8-bit quantized nodes
 	int cull = 1;             // 0 exact (no distance culling), 1 distance culling with margin
 	int block = 128;          // threads per workgroup
-	int waves_per_cu = 24;    // persistent waves per CU
-	int lds_stack = 16;       // LDS stack entries per thread (power of two)
+	int waves_per_cu = 32;    // persistent waves per CU
+	int lds_stack = 8;        // LDS stack entries per thread (power of two)
 	int chunk = 128;          // rays a wave reserves per atomic (refilled lane by lane)
 	int leaf_phase = 24;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int refill = 8;           // load new rays once at least this many lanes of a wave are idle

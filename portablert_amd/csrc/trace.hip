@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 #include "internal.h"
@@ -138,16 +139,22 @@ __device__ __forceinline__ float child_slab(const Ray &r, const float lo[3], con
 }
 
 // LDS ring stack: the newest `S` entries of each lane stay in LDS, older ones are moved to
-// the lane's spill region (entry at depth k lives at spill[k] once evicted). An entry is
+// global memory (the entry at depth k of launched thread g lives at spill[(k - S) * threads + g]
+// once evicted, so lanes spilling at equal depths write adjacent words). An entry is
 // {work entry, sort key of its box entry distance}.
 template <int S, int B>
 struct Stack {
-	uint2 *lds;    // [S][B]
-	uint2 *spill;  // this lane's spill region
+	uint2 *lds;        // [S][B]
+	uint2 *spill;      // shared spill array
+	uint32_t thread;   // launched thread index g
+	uint32_t threads;  // launched threads
 	uint32_t sp;
+	__device__ __forceinline__ uint2 *spilled(uint32_t depth) const {
+		return spill + (static_cast<uint64_t>(depth - S) * threads + thread);
+	}
 	__device__ __forceinline__ void push(uint32_t ref, uint32_t key) {
 		uint2 *slot = lds + (sp & (S - 1)) * B;
-		if (sp >= static_cast<uint32_t>(S)) spill[sp - S] = *slot;
+		if (sp >= static_cast<uint32_t>(S)) *spilled(sp) = *slot;
 		*slot = make_uint2(ref, key);
 		++sp;
 	}
@@ -155,7 +162,7 @@ struct Stack {
 		--sp;
 		uint2 *slot = lds + (sp & (S - 1)) * B;
 		const uint2 e = *slot;
-		if (sp >= static_cast<uint32_t>(S)) *slot = spill[sp - S];
+		if (sp >= static_cast<uint32_t>(S)) *slot = *spilled(sp);
 		return e;
 	}
 };
@@ -166,10 +173,15 @@ struct Stack {
 // clamped to -FLT_MAX, which every culling limit admits. Misses sort last.
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t sort_key(float tmin, int slot) {
-	const uint32_t u = __float_as_uint(fmaxf(tmin, -3.402823466e38f));
+// Key of a finite (or +inf) entry distance.
+__device__ __forceinline__ uint32_t sort_key_finite(float tmin, int slot) {
+	const uint32_t u = __float_as_uint(tmin);
 	const uint32_t o = u ^ (static_cast<uint32_t>(static_cast<int32_t>(u) >> 31) | 0x80000000u);
 	return (o & ~7u) | static_cast<uint32_t>(slot);
+}
+
+__device__ __forceinline__ uint32_t sort_key(float tmin, int slot) {
+	return sort_key_finite(fmaxf(tmin, -3.402823466e38f), slot);
 }
 
 // Largest key whose decoded distance is within `limit` (not NaN): a key k passes iff
@@ -241,6 +253,8 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 	// builder already rounded outward). fast_ok guarantees |1/d|, |o| and the scene coordinates
 	// are below 2^60, so none of these values overflows; other rays take the reference formula.
 	if (fast_ok) {
+		// a slot is used iff its meta byte is not 0xFF, i.e. its byte of the complement is not 0
+		const uint32_t unused_lo = ~meta_lo, unused_hi = ~meta_hi;
 		float step[3], bnear[3], bfar[3];
 		uint32_t qn[3][2], qf[3][2];
 #pragma unroll
@@ -259,7 +273,7 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 		}
 #pragma unroll
 		for (int k = 0; k < 8; ++k) {
-			const uint32_t meta = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+			const uint32_t used = ((k < 4 ? unused_lo : unused_hi) >> (8 * (k & 3))) & 0xFFu;
 			float tn[3], tf[3];
 #pragma unroll
 			for (int ax = 0; ax < 3; ++ax) {
@@ -268,8 +282,9 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			}
 			const float tmin = fmaxf(fmaxf(tn[0], tn[1]), tn[2]);
 			const float tmax = fminf(fminf(tf[0], tf[1]), tf[2]);
-			const bool hit = meta != kMetaEmpty && tmax >= 0.0f && tmin <= fminf(tmax, limit);
-			h.key[k] = hit ? sort_key(tmin, k) : kMiss;
+			// the fast bounds are finite (no overflow below 2^60), so no clamp in the key
+			const bool hit = used != 0u && tmax >= 0.0f && tmin <= tmax && tmin <= limit;
+			h.key[k] = hit ? sort_key_finite(tmin, k) : kMiss;
 		}
 		sort8(h.key);
 		return;
@@ -324,6 +339,14 @@ __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_
 	sort8(hh.key);
 }
 
+// Lane 0's value in every lane, as a wave-uniform (scalar) value; every lane of the wave must be
+// active.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+	const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+	const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+	return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
 // Writes the finished ray's record in the caller's layout.
 __device__ __forceinline__ void write_record(const KArgs &a, uint64_t ray_idx, const Ray &r) {
 	Hit h;
@@ -354,16 +377,25 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 	}
 }
 
+// Waves per SIMD the register allocation must allow. Quantized nodes with an 8-entry LDS stack
+// (4 KB per wave): 8, i.e. at most 64 VGPRs; the traversal is partly latency-bound (dependent
+// node fetches) and the extra waves hide more of it than the few spilled registers cost. A
+// 16-entry stack allows 5 waves by LDS alone. The fp32 node test needs about 100 VGPRs and is
+// left to the compiler (1 = no constraint).
+template <class Node, int S>
+constexpr int kWavesPerEu = std::is_same<Node, NodeQ>::value ? (S == 8 ? 8 : 5) : 1;
+
 template <class Node, int S, int B, bool COUNT, int MODE>
-__global__ __launch_bounds__(B) void k_trace(KArgs a) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<Node, S>))) void k_trace(KArgs a) {
 	constexpr bool SEEDED = MODE == kSeeded, BATCH = MODE == kBatch;
 	__shared__ uint2 lds[S * B];
 	const Node *nodes = static_cast<const Node *>(a.nodes);
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint64_t lt_mask = lane ? (~0ull >> (64u - lane)) : 0ull;
 	Stack<S, B> st;
 	st.lds = lds + threadIdx.x;
-	st.spill = a.spill + static_cast<uint64_t>(blockIdx.x * B + threadIdx.x) * a.spill_depth;
+	st.spill = a.spill;
+	st.thread = blockIdx.x * B + threadIdx.x;
+	st.threads = gridDim.x * B;
 	st.sp = 0;
 
 	Ray r;
@@ -391,7 +423,7 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 			if (chunk_next >= chunk_end && BATCH) {
 				uint64_t tix = 0;
 				if (lane == 0) tix = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter), 1ull);
-				tix = __shfl(tix, 0);
+				tix = uniform64(tix);
 				if (tix < a.n_tiles) {
 					const uint4 tl = a.tiles[tix];
 					chunk_next = static_cast<uint64_t>(tl.x) | (static_cast<uint64_t>(tl.y) << 32);
@@ -405,13 +437,15 @@ __global__ __launch_bounds__(B) void k_trace(KArgs a) {
 				uint64_t base = 0;
 				if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long *>(a.counter),
 				                                static_cast<unsigned long long>(a.chunk));
-				base = __shfl(base, 0);
+				base = uniform64(base);
 				chunk_next = base;
 				chunk_end = base + a.chunk < a.n_rays ? base + a.chunk : a.n_rays;
 				if (base >= a.n_rays) exhausted = true;
 			}
 			const uint64_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0ull;
-			const uint32_t rank = __popcll(need & lt_mask);
+			// idle lanes below this one
+			const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+			                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
 			if (!active && rank < avail) {
 				ray_idx = chunk_next + rank;
 				active = true;

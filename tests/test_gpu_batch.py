@@ -190,4 +190,4 @@ def test_tuner_keeps_results(gpu_ctx, batch_workload):
     assert torch.equal(again, eager)
     for k in range(4):  # back to defaults for later tests
         batch.set_params(k, (8, 2, 64))
-    batch.set_params(-1, (128, 24, 16))
+    batch.set_params(-1, (128, 32, 8))
