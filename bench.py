@@ -461,13 +461,16 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     vf = os.path.join(ROOT, "profiles", "valu_room.json")
     if os.path.exists(vf) and args.format == 1:
-        # the bound that holds: VALU (ROCm's VALUBusy from the committed counters of this workload)
         with open(vf) as f:
             vj = json.load(f)
-        valu = {"bound": "valu", "metric": "VALUBusy (rocprofv3 counters of this workload, profiles/valu_room.json)",
-                "frac": vj["valu_busy"], "lane_activity": vj["lane_activity"],
-                "achieved": round(vj["sq_insts_valu_per_dispatch"] / (kernel_ms * 1e-3) / 1e9, 1),
-                "unit": "G wave-instructions/s"}
+        # the bound that holds: VALU issue. Instructions per launch from the committed counters of
+        # this workload; the peak weights the probe's int32 and fp32 issue rates by the measured
+        # int32 share (gfx950 issues 32-bit integer VALU work about 1.8x faster than fp32)
+        achieved_valu = vj["sq_insts_valu_per_dispatch"] / (kernel_ms * 1e-3) / 1e9
+        valu = {"bound": "valu_issue", "achieved": round(achieved_valu, 1), "peak": vj["peak_for_mix_g_per_s"],
+                "unit": "G wave-instructions/s", "frac": round(achieved_valu / vj["peak_for_mix_g_per_s"], 4),
+                "int32_share": vj["int32_share"], "lane_activity": vj["lane_activity"],
+                "source": "profiles/valu_room.json (rocprofv3 counters, portablert_amd/lib/valu_rate probe)"}
     out = {
         "metric": METRIC,
         "value": round(value, 2),
