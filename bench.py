@@ -206,6 +206,48 @@ def precision_leg(ctx, d_rays, n, mask, d_ref, stream, steps):
     return res
 
 
+def refit_leg(ctx, tris, d_rays, n, mask, stream, steps):
+    """Dynamic geometry on the room: the triangles moved (every vertex displaced uniformly by up
+    to 1 cm, then up to 50 cm), the tree refit instead of rebuilt. Reports refit and build device
+    times and the traversal time on each tree; records of the refit tree must equal the rebuilt
+    tree's."""
+    import numpy as np
+    import torch
+    sp = stream.cuda_stream
+    rng = np.random.default_rng(11)
+    d_tris = torch.from_numpy(tris).cuda()
+    torch.cuda.synchronize()  # the library reads it on its own stream
+    ctx.set_tris_device(d_tris.data_ptr(), len(tris))
+    out_r = torch.empty(n * ctx_layout_size(mask), dtype=torch.uint8, device="cuda")
+    out_b = torch.empty_like(out_r)
+    res = {"workload": "room, 262,144 triangles, 8,388,608 rays; vertices displaced uniformly by up to d"}
+    for name, d in (("moved_1cm", 0.01), ("moved_50cm", 0.5)):
+        moved = (tris + rng.uniform(-d, d, tris.shape)).astype(np.float32)
+        d_moved = torch.from_numpy(moved).cuda()
+        torch.cuda.synchronize()
+        ctx.set_tris_device(d_tris.data_ptr(), len(tris))  # refit always starts from the original tree
+        refit = []
+        for _ in range(3):
+            ctx.refit_device(d_moved.data_ptr(), len(moved))
+            refit.append(ctx.stat("refit_ms"))
+        ms_r = _timed(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_r.data_ptr(), sp), steps, 1, stream)
+        ctx.set_tris_device(d_moved.data_ptr(), len(moved))
+        build = ctx.stat("build_ms")
+        ms_b = _timed(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_b.data_ptr(), sp), steps, 1, stream)
+        res[name] = {"refit_ms": round(float(np.median(refit)), 3), "build_ms": round(build, 3),
+                     "trace_ms_refit_tree": round(ms_r, 4), "trace_ms_rebuilt_tree": round(ms_b, 4),
+                     "identical": bool(torch.equal(out_r, out_b))}
+        log(f"refit {name}: refit {res[name]['refit_ms']:.3f} ms vs build {build:.3f} ms; traversal "
+            f"{ms_r:.3f} ms (refit tree) vs {ms_b:.3f} ms (rebuilt); identical {res[name]['identical']}")
+    ctx.set_tris_device(d_tris.data_ptr(), len(tris))
+    return res
+
+
+def ctx_layout_size(mask):
+    import portablert_amd as prt
+    return prt.layout(mask)[0]
+
+
 def _timed(fn, steps, warmup, stream):
     """Mean ms per call of fn over `steps` calls, bracketed by events on `stream`."""
     import torch
@@ -358,6 +400,7 @@ def main():
                     help="train the learned leg on the first N of the 16,777,216 training rays")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--no-icosphere", action="store_true", help="skip the icosphere primary + bounce leg")
+    ap.add_argument("--no-refit", action="store_true", help="skip the dynamic-geometry (refit) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU)",
@@ -452,6 +495,7 @@ def main():
 
     # precision: fp32 BVH8 nodes against 8-bit quantized nodes, same records
     precision = precision_leg(ctx, d_rays, n, mask, d_out, stream, max(5, args.steps // 2))
+    refit = None if args.no_refit else refit_leg(ctx, tris, d_rays, n, mask, stream, max(3, args.steps // 4))
 
     achieved = BYTES_PER_RAY * n / (kernel_ms * 1e-3) / 1e9
     traffic, valu = None, None
@@ -504,6 +548,8 @@ def main():
         "api_boundary_mrays_s": round(n / api_ms / 1e3, 2),
         "precision": precision,
     }
+    if refit is not None:
+        out["refit"] = refit
     if not args.no_learned:
         lr = learned_leg(ctx, tris, max(3, args.steps // 4), 1, args.epochs, args.train_rays)
         out["learned_mrays_s"] = lr["learned_mrays_s"]

@@ -45,6 +45,19 @@ class MI355XBackend : public InvokableBackend<MI355XBackend> {
 		check(prt_mi355x_set_tris(ctx_, tris.empty() ? nullptr : tris.data()->data(), tris.size()));
 	}
 
+	// Move version (the reference's backend.hpp:18 to-do): the triangles live on the device
+	// after the build, so the caller's host copy is released here.
+	void set_tris(Tris &&tris) {
+		set_tris(static_cast<const Tris &>(tris));
+		Tris().swap(tris);
+	}
+
+	// Same triangles at new positions: keeps the tree, recomputes its boxes (prt_mi355x_refit).
+	void refit(const Tris &tris) {
+		if (!ctx_) throw std::runtime_error("MI355X: refit before set_tris");
+		check(prt_mi355x_refit(ctx_, tris.empty() ? nullptr : tris.data()->data(), tris.size()));
+	}
+
 	std::string device_name() const override {
 		char buf[256] = {0};
 		if (ctx_) {

@@ -75,7 +75,8 @@ enum {
 	PRT_STAT_TRAIN_LOSS = 17,  /* cross-entropy of the last training batch */
 	PRT_STAT_CLUSTERS = 18,    /* BVH8 subtree clusters */
 	PRT_STAT_SEED_TOP1 = 19,   /* last learned_eval: fraction of rays whose top-1 cluster is true */
-	PRT_STAT_SEED_TOP4 = 20    /* ... whose true cluster is among the top 4 */
+	PRT_STAT_SEED_TOP4 = 20,   /* ... whose true cluster is among the top 4 */
+	PRT_STAT_REFIT_MS = 21     /* last refit, device time */
 };
 
 /* Number of usable gfx950 devices and the name of device 0 (name may be NULL). Returns PRT_OK
@@ -99,6 +100,19 @@ int prt_mi355x_set_tris(prt_ctx *ctx, const float *tris9, uint64_t n);
 
 /* Same with the triangles already in device memory (read during the call only). */
 int prt_mi355x_set_tris_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
+
+/* Refit for dynamic geometry: the same n triangles (same order and ids) at new positions. Keeps
+ * the tree's topology and recomputes every leaf record and node box (fp32 and quantized), so a
+ * frame of moving geometry costs a few launches instead of a build; records equal those of
+ * set_tris on the new positions (tree quality degrades as triangles move far). Errors:
+ * PRT_ERR_STATE before set_tris, PRT_ERR_ARG when n differs from the tree's count,
+ * PRT_ERR_INPUT for a non-finite vertex (the scene is then cleared). Learned seeding must be
+ * set up again afterwards. No counterpart in the reference, which rebuilds on every set_tris
+ * and notes a move version as a to-do (backend.hpp:18-19, intersect_cpu.cpp:12). */
+int prt_mi355x_refit(prt_ctx *ctx, const float *tris9, uint64_t n);
+
+/* Same with the triangles in device memory. */
+int prt_mi355x_refit_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
 
 /* Nearest hit of n rays in host memory, records written to host memory `out` with the
  * HitReg layout of `mask`: `stride` = sizeof(HitReg), off[8] = offsets of u v t primitive_id

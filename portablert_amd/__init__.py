@@ -24,7 +24,7 @@ OK, ERR_ARG, ERR_NODEV, ERR_HIP, ERR_OOM, ERR_STATE, ERR_LAYOUT, ERR_INPUT = ran
 OPT = dict(node_format=1, cull=2, leaf_size=3, block=4, waves_per_cu=5, lds_stack=6, chunk=7, leaf_phase=8,
            count_stats=9, time_kernel=10, alloc_limit=11, margin_rel_log2=12, builder=13, refill=14)
 STAT = dict(build_ms=1, trace_ms=2, nodes=3, depth=4, node_visits=5, tri_tests=6, leaf_visits=7,
-            node_bytes=8, api_ms=9, n_tris=10, node_phases=11, tri_phases=12, culled_pops=13)
+            node_bytes=8, api_ms=9, n_tris=10, node_phases=11, tri_phases=12, culled_pops=13, refit_ms=21)
 
 _lib = None
 
@@ -60,6 +60,8 @@ def lib():
         L.prt_mi355x_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
         L.prt_mi355x_set_tris.argtypes = [vp, vp, u64]
         L.prt_mi355x_set_tris_device.argtypes = [vp, vp, u64]
+        L.prt_mi355x_refit.argtypes = [vp, vp, u64]
+        L.prt_mi355x_refit_device.argtypes = [vp, vp, u64]
         L.prt_mi355x_nearest_hits.argtypes = [vp, vp, u64, u32, vp, u32, vp]
         L.prt_mi355x_nearest_hits_device.argtypes = [vp, vp, u64, u32, vp, u32, vp, vp]
         L.prt_mi355x_synchronize.argtypes = [vp]
@@ -157,6 +159,14 @@ class Context:
 
     def set_tris_device(self, ptr, n):
         self._check(lib().prt_mi355x_set_tris_device(self.h, ctypes.c_void_p(ptr), n))
+
+    def refit(self, tris):
+        """The same triangles at new positions: keeps the tree, recomputes its boxes."""
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self._check(lib().prt_mi355x_refit(self.h, _p(tris), len(tris)))
+
+    def refit_device(self, ptr, n):
+        self._check(lib().prt_mi355x_refit_device(self.h, ctypes.c_void_p(ptr), n))
 
     def nearest_hits(self, rays, mask=mask_of(ALL_TAGS)):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)

@@ -156,6 +156,41 @@ int prt_mi355x_device_name(const prt_ctx *c, char *buf, size_t cap) {
 	return PRT_OK;
 }
 
+// Refit from triangles already in c->bs.tri_in.
+static int do_refit(prt_ctx *c) {
+	bool bad = false;
+	double ms = 0.0;
+	const hipError_t e = prt::refit_bvh8(c->bs, c->bvh, c->stream, &bad, &ms);
+	if (e != hipSuccess) {
+		c->has_scene = false;
+		return hip_fail(c, e, "BVH refit");
+	}
+	if (bad) {
+		c->has_scene = false;
+		return fail(c, PRT_ERR_INPUT, "refit: a triangle vertex is not finite");
+	}
+	c->refit_ms = ms;
+	++c->scene_gen;  // boxes changed: learned clusters must be set up again
+	apply_margin(c);
+	return PRT_OK;
+}
+
+static int refit_common(prt_ctx *c, const float *tris, uint64_t n, hipMemcpyKind kind) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (!c->has_scene) return fail(c, PRT_ERR_STATE, "refit called before set_tris");
+	if (n != c->bvh.n_tris)
+		return fail(c, PRT_ERR_ARG, "refit: " + std::to_string(n) + " triangles, the tree has " + std::to_string(c->bvh.n_tris));
+	if (n == 0) return PRT_OK;
+	if (!tris) return fail(c, PRT_ERR_ARG, "refit: null triangle pointer");
+	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+	hipError_t e = prt::wait_use(c);  // traversals on caller streams still read the old boxes
+	if (e == hipSuccess) e = c->bs.tri_in.ensure(n * 36ull);
+	if (e == hipSuccess) e = hipMemcpyAsync(c->bs.tri_in.ptr, tris, n * 36ull, kind, c->stream);
+	if (e != hipSuccess) return hip_fail(c, e, "refit: triangle copy");
+	return do_refit(c);
+}
+
 int prt_mi355x_set_tris(prt_ctx *c, const float *tris, uint64_t n) {
 	if (!c) return PRT_ERR_ARG;
 	c->err.clear();
@@ -170,6 +205,14 @@ int prt_mi355x_set_tris(prt_ctx *c, const float *tris, uint64_t n) {
 		if (e != hipSuccess) return hip_fail(c, e, "set_tris: copy");
 	}
 	return do_build(c, n);
+}
+
+int prt_mi355x_refit(prt_ctx *c, const float *tris, uint64_t n) {
+	return refit_common(c, tris, n, hipMemcpyHostToDevice);
+}
+
+int prt_mi355x_refit_device(prt_ctx *c, const float *d_tris, uint64_t n) {
+	return refit_common(c, d_tris, n, hipMemcpyDeviceToDevice);
 }
 
 int prt_mi355x_set_tris_device(prt_ctx *c, const float *d_tris, uint64_t n) {
@@ -338,6 +381,7 @@ int prt_mi355x_get_stat(const prt_ctx *c, int stat, double *v) {
 	if (!c || !v) return PRT_ERR_ARG;
 	switch (stat) {
 	case PRT_STAT_BUILD_MS: *v = c->build_ms; break;
+	case PRT_STAT_REFIT_MS: *v = c->refit_ms; break;
 	case PRT_STAT_TRACE_MS: *v = c->trace_ms; break;
 	case PRT_STAT_NODES: *v = c->bvh.n_nodes; break;
 	case PRT_STAT_DEPTH: *v = c->bvh.depth; break;

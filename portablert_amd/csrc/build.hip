@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "internal.h"
@@ -869,6 +870,111 @@ __global__ __launch_bounds__(kBlock) void k_scene_boxes(const uint32_t *misc, ui
 
 inline uint32_t blocks_for(uint64_t n, int per = kBlock) { return static_cast<uint32_t>((n + per - 1) / per); }
 
+// ------------------------------------------------------------ refit: new vertices, same tree
+//
+// Every leaf record takes its primitive's new vertices; then every node's child boxes are
+// recomputed from the deepest level up (children have larger indices than their parent, and
+// a level's nodes are one contiguous index range), and the quantized boxes re-derived with the
+// build's outward rounding. Topology, child order and leaf membership do not change.
+
+__global__ __launch_bounds__(kBlock) void k_refit_tris(const float *__restrict__ tri_in, TriRec *__restrict__ tris,
+                                                       uint32_t n, uint32_t *misc) {
+	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	if (j >= n) return;
+	TriRec r = tris[j];
+	if (r.id >= n) {  // not a record of this tree: report, never read
+		atomicOr(misc + M_ERR, 2u);
+		return;
+	}
+	bool finite = true;
+#pragma unroll
+	for (int k = 0; k < 9; ++k) {
+		r.v[k] = tri_in[9ull * r.id + k];
+		finite &= __builtin_isfinite(r.v[k]);
+	}
+	tris[j] = r;
+	if (!finite) atomicOr(misc + M_ERR, 1u);
+}
+
+struct RefitArgs {
+	const TriRec *tris;
+	NodeQ *nq;
+	NodeF *nf;
+	float4 *box;  // [2 * n_nodes] node boxes (lo, hi), written here for the parent level
+	uint32_t n_nodes;
+	uint32_t n_tris;
+	uint32_t *misc;
+};
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(RefitArgs a, uint32_t begin, uint32_t end) {
+	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
+		NodeF f = a.nf[i];
+		NodeQ q = a.nq[i];
+		float nlo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+		float nhi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+#pragma unroll
+		for (int k = 0; k < 8; ++k) {
+			const uint32_t m = f.meta[k];
+			float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+			float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+			if (meta_internal(m)) {
+				const uint32_t c = f.child_base + meta_offset(m);
+				if (c <= i || c >= a.n_nodes) {
+					atomicOr(a.misc + M_ERR, 2u);
+				} else {
+					const float4 cl = a.box[2ull * c], ch = a.box[2ull * c + 1];
+					lo[0] = cl.x; lo[1] = cl.y; lo[2] = cl.z;
+					hi[0] = ch.x; hi[1] = ch.y; hi[2] = ch.z;
+				}
+			} else if (meta_used(m)) {
+				const uint32_t first = f.tri_base + meta_offset(m), cnt = meta_count(m);
+				if (first + cnt > a.n_tris) {
+					atomicOr(a.misc + M_ERR, 2u);
+				} else {
+					for (uint32_t t = first; t < first + cnt; ++t)
+						for (int v = 0; v < 3; ++v)
+#pragma unroll
+							for (int ax = 0; ax < 3; ++ax) {
+								lo[ax] = fminf(lo[ax], a.tris[t].v[3 * v + ax]);
+								hi[ax] = fmaxf(hi[ax], a.tris[t].v[3 * v + ax]);
+							}
+				}
+			}
+#pragma unroll
+			for (int ax = 0; ax < 3; ++ax) {
+				f.lo[ax][k] = lo[ax];
+				f.hi[ax][k] = hi[ax];
+				nlo[ax] = fminf(nlo[ax], lo[ax]);
+				nhi[ax] = fmaxf(nhi[ax], hi[ax]);
+			}
+		}
+		a.box[2ull * i] = make_float4(nlo[0], nlo[1], nlo[2], 0.0f);
+		a.box[2ull * i + 1] = make_float4(nhi[0], nhi[1], nhi[2], 0.0f);
+		float sc[3];
+#pragma unroll
+		for (int ax = 0; ax < 3; ++ax) {
+			const int e = grid_exp(nlo[ax], nhi[ax]);
+			q.p[ax] = nlo[ax];
+			q.e[ax] = static_cast<int8_t>(e);
+			sc[ax] = exp2i(e);
+		}
+#pragma unroll
+		for (int k = 0; k < 8; ++k) {
+#pragma unroll
+			for (int ax = 0; ax < 3; ++ax) {
+				if (meta_used(f.meta[k])) {
+					quantize(f.lo[ax][k], f.hi[ax][k], nlo[ax], sc[ax], q.qlo[ax][k], q.qhi[ax][k]);
+				} else {
+					q.qlo[ax][k] = 255;
+					q.qhi[ax][k] = 0;
+				}
+			}
+		}
+		a.nf[i] = f;
+		a.nq[i] = q;
+	}
+}
+
 }  // namespace
 
 hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 &out, hipStream_t st,
@@ -891,6 +997,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	const uint32_t S = ss.count;
 	const bool fixed = ss.levels > 0;  // no host synchronisation: capturable in a graph
 	*bad_input = false;
+	out.levels.clear();
 	out.n_tris = n64;
 	out.n_nodes = 0;
 	out.depth = 0;
@@ -1051,6 +1158,94 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	out.n_nodes = hm[M_NODES];
 	out.depth = hm[M_DEPTH];
 	if (hm[M_TRIS] != n || hm[M_NODES] != hm[M_LVL_END]) return hipErrorUnknown;  // incomplete build
+	return hipSuccess;
+#undef PRT_TRY
+}
+
+hipError_t refit_bvh8(BuildScratch &s, Bvh8 &b, hipStream_t st, bool *bad_input, double *refit_ms) {
+#define PRT_TRY(x)                                                                                 \
+	do {                                                                                           \
+		hipError_t e_ = (x);                                                                       \
+		if (e_ != hipSuccess) return e_;                                                           \
+	} while (0)
+	*bad_input = false;
+	const uint64_t n = b.n_tris;
+	if (n == 0 || b.n_nodes == 0) return hipSuccess;
+	if (b.levels.empty()) {
+		// level ranges from the node headers: level 0 is the root; the next level ends after the
+		// last internal child of the current one
+		std::vector<NodeF> h(b.n_nodes);
+		PRT_TRY(hipMemcpyAsync(h.data(), b.nodes_f.ptr, sizeof(NodeF) * b.n_nodes, hipMemcpyDeviceToHost, st));
+		PRT_TRY(hipStreamSynchronize(st));
+		uint32_t begin = 0, end = 1;
+		while (begin < end) {
+			b.levels.push_back(begin);
+			uint32_t next = end;
+			for (uint32_t i = begin; i < end; ++i)
+				for (int k = 0; k < 8; ++k)
+					if (meta_internal(h[i].meta[k]))
+						next = std::max(next, h[i].child_base + meta_offset(h[i].meta[k]) + 1u);
+			if (next > b.n_nodes) {
+				b.levels.clear();
+				return hipErrorUnknown;  // a child outside the tree
+			}
+			begin = end;
+			end = next;
+		}
+		if (begin != b.n_nodes) {
+			b.levels.clear();
+			return hipErrorUnknown;  // nodes not reachable level by level
+		}
+	}
+	PRT_TRY(s.refit_box.ensure(sizeof(float4) * 2ull * b.n_nodes));
+	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * misc_words(1)));
+	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * misc_words(1)));
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	if (refit_ms) {
+		PRT_TRY(hipEventCreate(&ev0));
+		PRT_TRY(hipEventCreate(&ev1));
+		PRT_TRY(hipEventRecord(ev0, st));
+	}
+	uint32_t *misc = s.misc.as<uint32_t>();
+	hipLaunchKernelGGL(k_init_misc, dim3(blocks_for(misc_words(1))), dim3(kBlock), 0, st, misc, misc_words(1));
+	hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_in.as<float>(), b.tris.as<TriRec>(),
+	                   static_cast<uint32_t>(n), misc);
+	RefitArgs ra;
+	ra.tris = b.tris.as<TriRec>();
+	ra.nq = b.nodes_q.as<NodeQ>();
+	ra.nf = b.nodes_f.as<NodeF>();
+	ra.box = s.refit_box.as<float4>();
+	ra.n_nodes = b.n_nodes;
+	ra.n_tris = static_cast<uint32_t>(n);
+	ra.misc = misc;
+	for (size_t l = b.levels.size(); l-- > 0;) {
+		const uint32_t begin = b.levels[l], end = l + 1 < b.levels.size() ? b.levels[l + 1] : b.n_nodes;
+		hipLaunchKernelGGL(k_refit_level, dim3(std::min<uint32_t>(blocks_for(end - begin), 2048)), dim3(kBlock), 0, st, ra,
+		                   begin, end);
+	}
+	PRT_TRY(hipGetLastError());
+	if (refit_ms) PRT_TRY(hipEventRecord(ev1, st));
+	float4 root[2];
+	PRT_TRY(hipMemcpyAsync(root, ra.box, sizeof(root), hipMemcpyDeviceToHost, st));
+	PRT_TRY(hipMemcpyAsync(s.host_misc, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
+	PRT_TRY(hipStreamSynchronize(st));
+	if (refit_ms) {
+		float ms = 0.0f;
+		PRT_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+		*refit_ms = ms;
+		(void)hipEventDestroy(ev0);
+		(void)hipEventDestroy(ev1);
+	}
+	const uint32_t err = static_cast<const uint32_t *>(s.host_misc)[M_ERR];
+	*bad_input = (err & 1u) != 0;
+	if (*bad_input) return hipSuccess;  // the caller reports the input; the tree is not used
+	if (err & 2u) return hipErrorUnknown;
+	b.scene_box[0] = root[0].x;
+	b.scene_box[1] = root[0].y;
+	b.scene_box[2] = root[0].z;
+	b.scene_box[3] = root[1].x;
+	b.scene_box[4] = root[1].y;
+	b.scene_box[5] = root[1].z;
 	return hipSuccess;
 #undef PRT_TRY
 }

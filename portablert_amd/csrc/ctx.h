@@ -29,7 +29,7 @@ struct prt_ctx {
 	int time_kernel = 0;
 	int margin_log2 = 10;
 	int64_t alloc_limit = 0;
-	double build_ms = 0.0, trace_ms = 0.0, api_ms = 0.0;
+	double build_ms = 0.0, trace_ms = 0.0, api_ms = 0.0, refit_ms = 0.0;
 	prt::TraceStats stats;
 	// learned seeding
 	prt::Mlp mlp;

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "common.h"
 
@@ -48,6 +49,7 @@ struct Bvh8 {
 	DevBuf nodes_q;          // NodeQ[n_nodes]
 	DevBuf nodes_f;          // NodeF[n_nodes]
 	DevBuf tris;             // TriRec[n_tris], in leaf order
+	std::vector<uint32_t> levels;  // first node of each BVH8 level (found at the first refit)
 };
 
 // Scratch of the LBVH build, kept across set_tris calls.
@@ -64,6 +66,7 @@ struct BuildScratch {
 	DevBuf misc;        // counters and per-scene bounds
 	DevBuf scene_root;  // binary root node of every scene
 	DevBuf ploc;        // clustering builder: cluster lists, neighbours, flags
+	DevBuf refit_box;   // refit: float4[2 * n_nodes] node boxes
 	void *host_misc = nullptr;  // pinned mirror of the counters and scene 0's bounds
 };
 
@@ -100,6 +103,11 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n, const BuildParams &p, Bvh8 &o
 // Error bits of the last build on `s`: 1 non-finite vertex, 2 triangle not placed, 4 BVH8 levels
 // exhausted before the build finished. Synchronises st.
 hipError_t build_errors(BuildScratch &s, hipStream_t st, uint32_t *flags);
+
+// New vertex positions (s.tri_in, the same n triangles in input order) for the tree in `b`:
+// leaf records and every node box are recomputed bottom-up, the topology is kept. Sets
+// *bad_input for a non-finite vertex (the tree is then not usable). Synchronises st.
+hipError_t refit_bvh8(BuildScratch &s, Bvh8 &b, hipStream_t st, bool *bad_input, double *refit_ms);
 
 // Traversal tunables.
 struct TraceParams {

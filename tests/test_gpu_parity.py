@@ -228,14 +228,13 @@ def test_error_paths(gpu_ctx):
     fresh.close()
 
 
-def test_bvh_invariants(gpu_ctx):
-    """Exported BVH: every triangle once; quantized child boxes contain the fp32 ones; fp32
-    child boxes contain their triangles."""
-    tris = scenes.scene(2)
-    gpu_ctx.set_tris(tris)
-    nq = gpu_ctx.export("nodes_q")
-    nf = gpu_ctx.export("nodes_f")
-    tr = gpu_ctx.export("tris").view(np.uint32).reshape(-1, 12)
+def check_bvh(ctx, tris):
+    """Exported BVH: every triangle once with its own vertices; quantized child boxes contain the
+    fp32 ones; internal child boxes equal the union of the child's slots; leaf boxes contain
+    their triangles."""
+    nq = ctx.export("nodes_q")
+    nf = ctx.export("nodes_f")
+    tr = ctx.export("tris").view(np.uint32).reshape(-1, 12)
     ids = tr[:, 9]
     assert np.array_equal(np.sort(ids), np.arange(len(tris), dtype=np.uint32))
     assert np.array_equal(tr[:, :9].view(np.float32), tris[ids])
@@ -258,18 +257,15 @@ def test_bvh_invariants(gpu_ctx):
     dlo, dhi = dlo.astype(np.float32), dhi.astype(np.float32)
     u3 = np.repeat(used[:, None, :], 3, 1)
     assert (dlo[u3] <= lo[u3]).all() and (dhi[u3] >= hi[u3]).all()
-    # internal children: the parent's fp32 box of the child equals the union of the child
-    # node's own child boxes (both come from the same binary-tree boxes)
     cbase = f[:, 0:4].copy().view(np.uint32)[:, 0]
     inner = used & ((meta & 0x80) != 0)
     node, slot = np.nonzero(inner)
     child = cbase[node] + (meta[node, slot] & 0x7F)
-    assert (child < n_nodes).all()
+    assert (child < n_nodes).all() and (child > node).all()
     cu = used[child]
     clo = np.where(cu[:, None, :], lo[child], np.inf).min(axis=2)
     chi = np.where(cu[:, None, :], hi[child], -np.inf).max(axis=2)
     assert np.array_equal(clo, lo[node, :, slot]) and np.array_equal(chi, hi[node, :, slot])
-    # leaves: boxes contain their triangles
     tbase = f[:, 4:8].copy().view(np.uint32)[:, 0]
     v = tr[:, :9].view(np.float32).reshape(-1, 3, 3)
     for k in range(8):
@@ -278,8 +274,80 @@ def test_bvh_invariants(gpu_ctx):
         cnt, off = (m & 3).astype(np.int64) + 1, (m >> 2).astype(np.int64)
         nodes = np.flatnonzero(leaf)
         first = tbase[nodes].astype(np.int64) + off[nodes]
+        llo = np.full((len(nodes), 3), np.inf, np.float32)
+        lhi = np.full((len(nodes), 3), -np.inf, np.float32)
         for j in range(4):  # leaves hold 1..4 triangles
             sel = cnt[nodes] > j
             tv = v[first[sel] + j]
-            assert (tv.min(axis=1) >= lo[nodes[sel], :, k]).all()
-            assert (tv.max(axis=1) <= hi[nodes[sel], :, k]).all()
+            llo[sel] = np.minimum(llo[sel], tv.min(axis=1))
+            lhi[sel] = np.maximum(lhi[sel], tv.max(axis=1))
+        # leaf boxes are exactly the union of their triangles' boxes
+        assert np.array_equal(llo, lo[nodes, :, k]) and np.array_equal(lhi, hi[nodes, :, k])
+    return n_nodes
+
+
+def moved(tris, seed, scale):
+    """Every vertex displaced by up to `scale` (a deformation keeping triangle ids)."""
+    rng = np.random.default_rng(seed)
+    return (tris + rng.uniform(-scale, scale, tris.shape)).astype(np.float32)
+
+
+def test_refit_matches_rebuild(gpu_ctx):
+    """Refit to moved geometry: the tree keeps its nodes, every box is recomputed, and the
+    records equal the oracle's (and a fresh build's) on the new positions, for a small motion,
+    a large one, and geometry swapped for another scene of the same size."""
+    tris = scenes.scene(0)
+    rays = scenes.rays(0, first=200000, count=150000)
+    mask = prt.mask_of(prt.ALL_TAGS)
+    gpu_ctx.set_tris(tris)
+    n0 = check_bvh(gpu_ctx, tris)
+    fresh = prt.Context(0)
+    for step, new in enumerate((moved(tris, 1, 0.002), moved(tris, 2, 0.2),
+                                scenes.scene(2)[:len(tris)] * np.float32(0.1))):
+        gpu_ctx.refit(new)
+        assert check_bvh(gpu_ctx, new) == n0  # same tree
+        got = gpu_ctx.nearest_hits(rays, mask)
+        assert_same(got, oracle.trace(new, rays), what=f"refit step {step}")
+        fresh.set_tris(new)
+        assert got.tobytes() == fresh.nearest_hits(rays, mask).tobytes()
+        for fmt, cull in VARIANTS[1:]:
+            configure(gpu_ctx, fmt, cull)
+            assert got.tobytes() == gpu_ctx.nearest_hits(rays, mask).tobytes(), (step, fmt, cull)
+        configure(gpu_ctx)
+        print(f"refit step {step}: {gpu_ctx.stat('refit_ms'):.3f} ms", flush=True)
+    fresh.close()
+
+
+def test_refit_errors(gpu_ctx):
+    fresh = prt.Context(0)
+    tris = scenes.scene(0)[:1000]
+    with pytest.raises(prt.PrtError) as e:
+        fresh.refit(tris)
+    assert e.value.code == prt.ERR_STATE
+    fresh.set_tris(tris)
+    with pytest.raises(prt.PrtError) as e:
+        fresh.refit(tris[:999])
+    assert e.value.code == prt.ERR_ARG
+    bad = tris.copy()
+    bad[10, 2] = np.inf
+    with pytest.raises(prt.PrtError) as e:
+        fresh.refit(bad)
+    assert e.value.code == prt.ERR_INPUT
+    rays = scenes.rays(0, first=0, count=10)
+    with pytest.raises(prt.PrtError) as e:  # the scene was cleared
+        fresh.nearest_hits(rays)
+    assert e.value.code == prt.ERR_STATE
+    fresh.set_tris(tris)
+    fresh.refit(tris)
+    assert_same(fresh.nearest_hits(rays), oracle.trace(tris, rays), what="after refit errors")
+    fresh.close()
+
+
+def test_bvh_invariants(gpu_ctx):
+    """Exported BVH of the room: every triangle once; quantized child boxes contain the fp32
+    ones; internal child boxes are the union of the child's slots; leaf boxes are the union of
+    their triangles."""
+    tris = scenes.scene(2)
+    gpu_ctx.set_tris(tris)
+    check_bvh(gpu_ctx, tris)
+

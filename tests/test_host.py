@@ -66,6 +66,8 @@ def test_null_and_bad_arguments_need_no_device():
     out = np.zeros(9, np.int32)
     assert lib.prt_mi355x_layout(32, out.ctypes.data_as(ctypes.c_void_p)) == prt.ERR_ARG
     assert lib.prt_mi355x_set_tris(None, None, 0) == prt.ERR_ARG
+    assert lib.prt_mi355x_refit(None, None, 0) == prt.ERR_ARG
+    assert lib.prt_mi355x_refit_device(None, None, 0) == prt.ERR_ARG
     assert lib.prt_mi355x_nearest_hits(None, None, 0, 0, None, 0, None) == prt.ERR_ARG
     assert lib.prt_mi355x_last_error(None) == b"null context"
 
