@@ -166,16 +166,19 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 
 // ------------------------------------------------------------- fused inference with top-4
 //
-// One workgroup (4 waves) takes 128 rays through the whole network: encoding, the three hidden
-// layers and the 4097 output logits, keeping the activations in LDS (X) and streaming W^T tiles
-// of 128 output neurons (W, prefetched into registers one tile ahead) from L2. Every 128 x 128
-// tile is the same sequence of v_mfma_f32_16x16x32_bf16 over K in steps of 32 as k_gemm, so
-// the logits are bit-identical to the unfused path; the output tiles go through LDS (over W)
-// to two row-owner threads per ray that keep a running top-4 (value descending, index
-// ascending) across all 33 tiles, so no logit leaves the chip.
-constexpr int kFRows = 128;    // rays per workgroup
+// One workgroup (4 waves) takes 64 rays through the whole network: encoding, the three hidden
+// layers and the 4097 output logits, keeping the activations in LDS (X, 64 x 256 bf16) and
+// streaming W^T tiles of 64 output neurons (W, prefetched into registers one tile ahead) from
+// L2. Each wave computes a 32 x 32 part of every 64 x 64 tile with v_mfma_f32_16x16x32_bf16
+// over K in steps of 32. For the output layer each wave first loads its 32 rows of the last
+// hidden layer into registers, so the 66 output tiles read only W from LDS. Every output tile
+// goes through LDS (over W) to four row-owner threads per ray (16 columns each) that keep a
+// running top-4 (value descending, index ascending); no logit leaves the chip. The workgroup
+// uses 66 KB of LDS, so two share a CU: one's MFMAs overlap the other's top-4 scan.
+constexpr int kFRows = 64;     // rays per workgroup
+constexpr int kFCols = 64;     // output neurons per W tile
 constexpr int kFStride = 264;  // bf16 row stride of X and W (16-byte reads without bank conflicts)
-constexpr int kFCStride = 132; // fp32 row stride of the output tile over W
+constexpr int kFCStride = 68;  // fp32 row stride of the output tile over W
 
 struct FusedArgs {
 	const float *rays;
@@ -187,89 +190,80 @@ struct FusedArgs {
 	uint32_t *top4;        // [rows][4]
 };
 
-// Global -> registers: the 128 x K bf16 tile of W^T rows [n0, n0 + 128), 16-byte chunks,
+// Global -> registers: the 64 x K bf16 tile of W^T rows [n0, n0 + 64), 16-byte chunks,
 // chunk c = tid + 256 q is row c / (K / 8), column group c % (K / 8).
 template <int K>
-__device__ __forceinline__ void tile_fetch(const uint16_t *wt, int n0, int tid, u32x4 pre[K / 16]) {
+__device__ __forceinline__ void tile_fetch(const uint16_t *wt, int n0, int tid, u32x4 pre[K / 32]) {
 	constexpr int per_row = K / 8;
 #pragma unroll
-	for (int q = 0; q < K / 16; ++q) {
+	for (int q = 0; q < K / 32; ++q) {
 		const int c = tid + 256 * q;
 		pre[q] = *reinterpret_cast<const u32x4 *>(wt + static_cast<size_t>(n0 + c / per_row) * K + (c % per_row) * 8);
 	}
 }
 
 template <int K>
-__device__ __forceinline__ void tile_store(uint16_t *W, int tid, const u32x4 pre[K / 16]) {
+__device__ __forceinline__ void tile_store(uint16_t *W, int tid, const u32x4 pre[K / 32]) {
 	constexpr int per_row = K / 8;
 #pragma unroll
-	for (int q = 0; q < K / 16; ++q) {
+	for (int q = 0; q < K / 32; ++q) {
 		const int c = tid + 256 * q;
 		*reinterpret_cast<u32x4 *>(W + (c / per_row) * kFStride + (c % per_row) * 8) = pre[q];
 	}
 }
 
-// acc (this wave's 64 x 64 part at rows wr, columns wc) = X[:, :K] * W[:, :K]^T
+// acc (this wave's 32 x 32 part at rows wr, columns wc) = X[:, :K] * W[:, :K]^T
 template <int K>
 __device__ __forceinline__ void tile_mma(const uint16_t *X, const uint16_t *W, int wr, int wc, int lane,
-                                         f32x4 acc[4][4]) {
+                                         f32x4 acc[2][2]) {
 #pragma unroll
-	for (int i = 0; i < 4; ++i)
+	for (int i = 0; i < 2; ++i)
 #pragma unroll
-		for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
+		for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
 	for (int k0 = 0; k0 < K; k0 += kBK) {
-		bf16x8 af[4], bfr[4];
+		bf16x8 af[2], bfr[2];
 #pragma unroll
-		for (int s = 0; s < 4; ++s) {
+		for (int s = 0; s < 2; ++s) {
 			af[s] = *reinterpret_cast<const bf16x8 *>(X + (wr + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
 			bfr[s] = *reinterpret_cast<const bf16x8 *>(W + (wc + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
 		}
 #pragma unroll
-		for (int i = 0; i < 4; ++i)
+		for (int i = 0; i < 2; ++i)
 #pragma unroll
-			for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+			for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
 	}
 }
 
-// X[row][n0 + col] = bf16(relu(acc + bias)) for this wave's part of a 128 x 128 tile
-__device__ __forceinline__ void tile_relu_store(uint16_t *X, const f32x4 acc[4][4], const float *bias, int n0, int wr,
-                                                int wc, int lane) {
-#pragma unroll
-	for (int j = 0; j < 4; ++j) {
-		const int col = n0 + wc + j * 16 + (lane & 15);
-		const float bj = bias[col];
-#pragma unroll
-		for (int i = 0; i < 4; ++i)
-#pragma unroll
-			for (int e = 0; e < 4; ++e)
-				X[(wr + i * 16 + (lane >> 4) * 4 + e) * kFStride + col] = f2bf(fmaxf(acc[i][j][e] + bj, 0.0f));
-	}
-}
-
-// One hidden layer over the 128 rays of X (input width K, 256 outputs).
+// One hidden layer over the 64 rays of X (input width K, 256 outputs in four tiles of 64; the
+// four accumulator sets stay in registers until X can be overwritten).
 template <int K>
 __device__ __forceinline__ void hidden_layer(uint16_t *X, uint16_t *W, const uint16_t *wt, const float *bias, int tid,
                                              int wr, int wc, int lane) {
-	f32x4 acc0[4][4], acc1[4][4];
-	{
-		u32x4 pre[K / 16];
-		tile_fetch<K>(wt, 0, tid, pre);
-		tile_store<K>(W, tid, pre);
+	f32x4 acc[4][2][2];
+#pragma unroll
+	for (int t = 0; t < 4; ++t) {
+		{
+			u32x4 pre[K / 32];
+			tile_fetch<K>(wt, t * kFCols, tid, pre);
+			tile_store<K>(W, tid, pre);
+		}
+		__syncthreads();
+		tile_mma<K>(X, W, wr, wc, lane, acc[t]);
+		__syncthreads();
 	}
-	__syncthreads();
-	tile_mma<K>(X, W, wr, wc, lane, acc0);
-	__syncthreads();
-	{
-		u32x4 pre[K / 16];
-		tile_fetch<K>(wt, 128, tid, pre);
-		tile_store<K>(W, tid, pre);
-	}
-	__syncthreads();
-	tile_mma<K>(X, W, wr, wc, lane, acc1);
-	__syncthreads();
-	tile_relu_store(X, acc0, bias, 0, wr, wc, lane);
-	tile_relu_store(X, acc1, bias, 128, wr, wc, lane);
+#pragma unroll
+	for (int t = 0; t < 4; ++t)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) {
+			const int col = t * kFCols + wc + j * 16 + (lane & 15);
+			const float bj = bias[col];
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int e = 0; e < 4; ++e)
+					X[(wr + i * 16 + (lane >> 4) * 4 + e) * kFStride + col] = f2bf(fmaxf(acc[t][i][j][e] + bj, 0.0f));
+		}
 	__syncthreads();
 }
 
@@ -291,15 +285,28 @@ __device__ __forceinline__ void top_insert(float v[4], uint32_t ix[4], float x, 
 	}
 }
 
-__global__ __launch_bounds__(256, 1) void k_mlp_top4(FusedArgs a) {
+// Merges the list of the lane `mask` away into this lane's list.
+__device__ __forceinline__ void top_merge(float v[4], uint32_t ix[4], int mask) {
+	float ov[4];
+	uint32_t oi[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		ov[q] = __shfl_xor(v[q], mask);
+		oi[q] = __shfl_xor(ix[q], mask);
+	}
+#pragma unroll
+	for (int q = 0; q < 4; ++q) top_insert(v, ix, ov[q], oi[q]);
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	__shared__ __align__(16) uint16_t X[kFRows * kFStride];
-	__shared__ __align__(16) uint16_t W[kFRows * kFStride];
-	float *C = reinterpret_cast<float *>(W);  // output tile [128][132] fp32, over W
+	__shared__ __align__(16) uint16_t W[kFCols * kFStride];
+	float *C = reinterpret_cast<float *>(W);  // output tile [64][68] fp32, over W
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+	const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 	const int r0 = blockIdx.x * kFRows;
 	// encoding: two threads per ray, three inputs each; rows past the end are zero
-	{
+	if (tid < 2 * kFRows) {
 		const int row = tid >> 1, half = tid & 1;
 		uint16_t *o = X + row * kFStride;
 		if (r0 + row < a.rows) {
@@ -308,45 +315,67 @@ __global__ __launch_bounds__(256, 1) void k_mlp_top4(FusedArgs a) {
 			for (int c = half * 96; c < half * 96 + 96; ++c) o[c] = 0;
 		}
 	}
-	// hidden layers: two tiles of 128 neurons each (loaded without prefetch, so only the two
-	// accumulator sets are live), then X is overwritten with the layer output
+	__syncthreads();
 	hidden_layer<kMlpIn>(X, W, a.w[0], a.b[0], tid, wr, wc, lane);
 	hidden_layer<kMlpHidden>(X, W, a.w[1], a.b[1], tid, wr, wc, lane);
 	hidden_layer<kMlpHidden>(X, W, a.w[2], a.b[2], tid, wr, wc, lane);
+	// output layer: this wave's 32 rows of the last hidden layer in registers
+	bf16x8 ar[2][kMlpHidden / kBK];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int kk = 0; kk < kMlpHidden / kBK; ++kk)
+			ar[i][kk] = *reinterpret_cast<const bf16x8 *>(X + (wr + i * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
 	{
-		u32x4 p0[kMlpHidden / 16];
+		u32x4 p0[kMlpHidden / 32];
 		tile_fetch<kMlpHidden>(a.w[3], 0, tid, p0);
 		tile_store<kMlpHidden>(W, tid, p0);
 	}
 	__syncthreads();
-	// Output layer: each tile (plus bias) goes through LDS (over W) to two row-owner threads per
-	// ray (columns 0-63 and 64-127), which keep a running top-4. A thread sees its columns in
-	// ascending order, so a strict comparison keeps the lower index first among equal logits.
-	const int orow = tid >> 1, ohalf = tid & 1;
+	// Each output tile (plus bias) goes through LDS (over W) to four row-owner threads per ray
+	// (16 columns each), which keep a running top-4. A thread sees its columns in ascending
+	// order, so a strict comparison keeps the lower index first among equal logits.
+	const int orow = tid >> 2, opart = tid & 3;
 	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-	constexpr int kOutTiles = kMlpOutPad / 128;
+	constexpr int kOutTiles = kMlpOutPad / kFCols;
 	for (int t = 0; t < kOutTiles; ++t) {
 		// next tile (the first again after the last: unconditional loads keep pre in registers);
 		// the loads overlap this tile's MFMAs and the top-4 scan
-		u32x4 pre[kMlpHidden / 16];
-		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * 128, tid, pre);
-		f32x4 acc[4][4];
-		tile_mma<kMlpHidden>(X, W, wr, wc, lane, acc);
+		u32x4 pre[kMlpHidden / 32];
+		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * kFCols, tid, pre);
+		f32x4 acc[2][2];
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+#pragma unroll
+			for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+		for (int kk = 0; kk < kMlpHidden / kBK; ++kk) {
+			bf16x8 bfr[2];
+#pragma unroll
+			for (int s = 0; s < 2; ++s)
+				bfr[s] = *reinterpret_cast<const bf16x8 *>(W + (wc + s * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr[j], acc[i][j], 0, 0, 0);
+		}
 		__syncthreads();  // every wave is done reading W
 #pragma unroll
-		for (int j = 0; j < 4; ++j) {
+		for (int j = 0; j < 2; ++j) {
 			const int col = wc + j * 16 + (lane & 15);
-			const float bj = a.b[3][t * 128 + col];
+			const float bj = a.b[3][t * kFCols + col];
 #pragma unroll
-			for (int i = 0; i < 4; ++i)
+			for (int i = 0; i < 2; ++i)
 #pragma unroll
 				for (int e = 0; e < 4; ++e) C[(wr + i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][j][e] + bj;
 		}
 		__syncthreads();
-		const float *crow = C + orow * kFCStride + ohalf * 64;
-		const int nbase = t * 128 + ohalf * 64;
-		for (int c4 = 0; c4 < 16; ++c4) {
+		const float *crow = C + orow * kFCStride + opart * 16;
+		const int nbase = t * kFCols + opart * 16;
+#pragma unroll
+		for (int c4 = 0; c4 < 4; ++c4) {
 			const float4 q = *reinterpret_cast<const float4 *>(crow + c4 * 4);
 			const float xs[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
@@ -369,18 +398,10 @@ __global__ __launch_bounds__(256, 1) void k_mlp_top4(FusedArgs a) {
 		tile_store<kMlpHidden>(W, tid, pre);
 		__syncthreads();
 	}
-	// merge the two halves of each ray (value descending, index ascending): insert the
-	// partner's four entries into this list
-	float ov[4];
-	uint32_t oi[4];
-#pragma unroll
-	for (int q = 0; q < 4; ++q) {
-		ov[q] = __shfl_xor(v[q], 1);
-		oi[q] = __shfl_xor(ix[q], 1);
-	}
-#pragma unroll
-	for (int q = 0; q < 4; ++q) top_insert(v, ix, ov[q], oi[q]);
-	if (ohalf == 0 && r0 + orow < a.rows) {
+	// merge the four parts of each ray (value descending, index ascending)
+	top_merge(v, ix, 1);
+	top_merge(v, ix, 2);
+	if (opart == 0 && r0 + orow < a.rows) {
 		const uint32_t res[4] = {ix[0], ix[1], ix[2], ix[3]};
 		reinterpret_cast<uint4 *>(a.top4)[r0 + orow] = make_uint4(res[0], res[1], res[2], res[3]);
 	}
