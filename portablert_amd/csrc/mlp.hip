@@ -172,9 +172,10 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 // L2. Each wave computes a 32 x 32 part of every 64 x 64 tile with v_mfma_f32_16x16x32_bf16
 // over K in steps of 32. For the output layer each wave first loads its 32 rows of the last
 // hidden layer into registers, so the 66 output tiles read only W from LDS. Every output tile
-// goes through LDS (over W) to four row-owner threads per ray (16 columns each) that keep a
-// running top-4 (value descending, index ascending); no logit leaves the chip. The workgroup
-// uses 66 KB of LDS, so two share a CU: one's MFMAs overlap the other's top-4 scan.
+// goes through LDS (over its W buffer; W is double-buffered in X's space once X's rows are in
+// registers) to four row-owner threads per ray (16 columns each) that keep a running top-4
+// (value descending, index ascending); no logit leaves the chip. The workgroup uses 66 KB of
+// LDS, so two share a CU: one's MFMAs overlap the other's top-4 scan.
 constexpr int kFRows = 64;     // rays per workgroup
 constexpr int kFCols = 64;     // output neurons per W tile
 constexpr int kFStride = 264;  // bf16 row stride of X and W (16-byte reads without bank conflicts)
@@ -301,7 +302,6 @@ __device__ __forceinline__ void top_merge(float v[4], uint32_t ix[4], int mask) 
 __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	__shared__ __align__(16) uint16_t X[kFRows * kFStride];
 	__shared__ __align__(16) uint16_t W[kFCols * kFStride];
-	float *C = reinterpret_cast<float *>(W);  // output tile [64][68] fp32, over W
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 	const int r0 = blockIdx.x * kFRows;
@@ -338,10 +338,12 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	const int orow = tid >> 2, opart = tid & 3;
 	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-	constexpr int kOutTiles = kMlpOutPad / kFCols;
+	constexpr int kOutTiles = (kMlpOut + kFCols - 1) / kFCols;  // the last one holds the miss logit
 	for (int t = 0; t < kOutTiles; ++t) {
-		// next tile (the first again after the last: unconditional loads keep pre in registers);
-		// the loads overlap this tile's MFMAs and the top-4 scan
+		// tile t is in Wt; the next goes to the other buffer (X's, free once its rows are in
+		// registers) during this tile's MFMAs. The output tile goes through LDS over Wt.
+		uint16_t *Wt = (t & 1) ? X : W, *Wn = (t & 1) ? W : X;
+		float *Ct = reinterpret_cast<float *>(Wt);
 		u32x4 pre[kMlpHidden / 32];
 		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * kFCols, tid, pre);
 		f32x4 acc[2][2];
@@ -354,49 +356,58 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 			bf16x8 bfr[2];
 #pragma unroll
 			for (int s = 0; s < 2; ++s)
-				bfr[s] = *reinterpret_cast<const bf16x8 *>(W + (wc + s * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+				bfr[s] = *reinterpret_cast<const bf16x8 *>(Wt + (wc + s * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
 #pragma unroll
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
 				for (int j = 0; j < 2; ++j)
 					acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr[j], acc[i][j], 0, 0, 0);
 		}
-		__syncthreads();  // every wave is done reading W
+		tile_store<kMlpHidden>(Wn, tid, pre);
+		__syncthreads();  // every wave is done reading Wt; the next tile is stored
 #pragma unroll
 		for (int j = 0; j < 2; ++j) {
 			const int col = wc + j * 16 + (lane & 15);
-			const float bj = a.b[3][t * kFCols + col];
+			// padding columns get -inf: never taken into a top-4
+			const float bj = t * kFCols + col < kMlpOut ? a.b[3][t * kFCols + col] : -__builtin_inff();
 #pragma unroll
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
-				for (int e = 0; e < 4; ++e) C[(wr + i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][j][e] + bj;
+				for (int e = 0; e < 4; ++e) Ct[(wr + i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][j][e] + bj;
 		}
 		__syncthreads();
-		const float *crow = C + orow * kFCStride + opart * 16;
+		const float *crow = Ct + orow * kFCStride + opart * 16;
 		const int nbase = t * kFCols + opart * 16;
+		// candidates: above this thread's 4th best, and not below the row's bound (the largest
+		// 4th best of the row's four threads: the row's final 4th best is at least that)
+		const float tau1 = fmaxf(v[3], __shfl_xor(v[3], 1));
+		const float tau = fmaxf(tau1, __shfl_xor(tau1, 2));
+		uint32_t cand = 0;
 #pragma unroll
 		for (int c4 = 0; c4 < 4; ++c4) {
 			const float4 q = *reinterpret_cast<const float4 *>(crow + c4 * 4);
 			const float xs[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-			for (int e = 0; e < 4; ++e) {
-				const int n = nbase + c4 * 4 + e;
-				if (xs[e] > v[3] && n < kMlpOut) {
-					v[3] = xs[e];
-					ix[3] = static_cast<uint32_t>(n);
+			for (int e = 0; e < 4; ++e) cand |= (xs[e] > v[3] && xs[e] >= tau) ? (1u << (c4 * 4 + e)) : 0u;
+		}
+		// insert them in column order (a strict comparison keeps the lower index first)
+		while (cand) {
+			const int c = __builtin_ctz(cand);
+			cand &= cand - 1u;
+			const float x = crow[c];
+			if (x > v[3]) {
+				v[3] = x;
+				ix[3] = static_cast<uint32_t>(nbase + c);
 #pragma unroll
-					for (int k = 3; k > 0; --k) {
-						if (v[k] > v[k - 1]) {
-							const float tv = v[k]; v[k] = v[k - 1]; v[k - 1] = tv;
-							const uint32_t ti = ix[k]; ix[k] = ix[k - 1]; ix[k - 1] = ti;
-						}
+				for (int k = 3; k > 0; --k) {
+					if (v[k] > v[k - 1]) {
+						const float tv = v[k]; v[k] = v[k - 1]; v[k - 1] = tv;
+						const uint32_t ti = ix[k]; ix[k] = ix[k - 1]; ix[k - 1] = ti;
 					}
 				}
 			}
 		}
-		__syncthreads();  // the tile in LDS is read
-		tile_store<kMlpHidden>(W, tid, pre);
-		__syncthreads();
+		__syncthreads();  // the output tile is read: Wt is free for tile t + 2
 	}
 	// merge the four parts of each ray (value descending, index ascending)
 	top_merge(v, ix, 1);
