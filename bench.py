@@ -209,8 +209,8 @@ def precision_leg(ctx, d_rays, n, mask, d_ref, stream, steps):
 def refit_leg(ctx, tris, d_rays, n, mask, stream, steps):
     """Dynamic geometry on the room: the triangles moved (every vertex displaced uniformly by up
     to 1 cm, then up to 50 cm), the tree refit instead of rebuilt. Reports refit and build device
-    times and the traversal time on each tree; records of the refit tree must equal the rebuilt
-    tree's."""
+    times and the median traversal time on each tree; records of the refit tree must equal the
+    rebuilt tree's."""
     import numpy as np
     import torch
     sp = stream.cuda_stream
@@ -230,10 +230,10 @@ def refit_leg(ctx, tris, d_rays, n, mask, stream, steps):
         for _ in range(3):
             ctx.refit_device(d_moved.data_ptr(), len(moved))
             refit.append(ctx.stat("refit_ms"))
-        ms_r = _timed(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_r.data_ptr(), sp), steps, 1, stream)
+        ms_r = _timed_median(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_r.data_ptr(), sp), steps, 1, stream)
         ctx.set_tris_device(d_moved.data_ptr(), len(moved))
         build = ctx.stat("build_ms")
-        ms_b = _timed(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_b.data_ptr(), sp), steps, 1, stream)
+        ms_b = _timed_median(lambda: ctx.nearest_hits_device(d_rays.data_ptr(), n, mask, out_b.data_ptr(), sp), steps, 1, stream)
         res[name] = {"refit_ms": round(float(np.median(refit)), 3), "build_ms": round(build, 3),
                      "trace_ms_refit_tree": round(ms_r, 4), "trace_ms_rebuilt_tree": round(ms_b, 4),
                      "identical": bool(torch.equal(out_r, out_b))}
@@ -246,6 +246,22 @@ def refit_leg(ctx, tris, d_rays, n, mask, stream, steps):
 def ctx_layout_size(mask):
     import portablert_amd as prt
     return prt.layout(mask)[0]
+
+
+def _timed_median(fn, steps, warmup, stream):
+    """Median ms of single calls of fn, each bracketed by its own events on `stream`."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
 def _timed(fn, steps, warmup, stream):
