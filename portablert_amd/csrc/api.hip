@@ -141,6 +141,16 @@ void prt_mi355x_destroy(prt_ctx *c) {
 	                       &c->out_stage};
 	for (auto *b : bufs) b->release();
 	if (c->bs.host_misc) (void)hipHostFree(c->bs.host_misc);
+	c->pool.reset();
+	for (int b = 0; b < 2; ++b) {
+		if (c->pin_in[b]) (void)hipHostFree(c->pin_in[b]);
+		if (c->pin_out[b]) (void)hipHostFree(c->pin_out[b]);
+		hipEvent_t evs[3] = {c->ev_in[b], c->ev_tr[b], c->ev_out[b]};
+		for (hipEvent_t ev : evs)
+			if (ev) (void)hipEventDestroy(ev);
+	}
+	if (c->s_in) (void)hipStreamDestroy(c->s_in);
+	if (c->s_out) (void)hipStreamDestroy(c->s_out);
 	if (c->ts.host_err) (void)hipHostFree(c->ts.host_err);
 	c->batch.reset();
 	if (c->busy) (void)hipEventDestroy(c->busy);
@@ -243,6 +253,85 @@ static int trace_common(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t ma
 	return PRT_OK;
 }
 
+// Rays per chunk of the host-buffer entry point.
+constexpr uint64_t kHostChunk = uint64_t{1} << 20;
+
+static hipError_t ensure_pinned(void *buf[2], size_t *have, size_t want) {
+	if (want <= *have) return hipSuccess;
+	for (int b = 0; b < 2; ++b) {
+		if (buf[b]) (void)hipHostFree(buf[b]);
+		buf[b] = nullptr;
+	}
+	*have = 0;
+	for (int b = 0; b < 2; ++b) {
+		const hipError_t e = hipHostMalloc(&buf[b], want);
+		if (e != hipSuccess) return e;
+	}
+	*have = want;
+	return hipSuccess;
+}
+
+// Host rays -> records in chunks: chunk i is copied into a pinned buffer by the copy threads
+// and sent to the device on s_in while the context's stream traces chunk i - 1 and s_out
+// brings its records back; the host copies chunk i - 1's records out of their pinned buffer
+// meanwhile. Each chunk's error flags are kept (every traversal launch clears them).
+static int nearest_hits_chunked(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride) {
+	const uint64_t K = (n + kHostChunk - 1) / kHostChunk;
+	hipError_t e = hipSuccess;
+	if (!c->pool) c->pool.reset(new prt::CopyPool(prt::CopyPool::default_threads()));
+	e = ensure_pinned(c->pin_in, &c->pin_in_bytes, kHostChunk * 24);
+	if (e == hipSuccess) e = ensure_pinned(c->pin_out, &c->pin_out_bytes, kHostChunk * stride);
+	if (e == hipSuccess && !c->s_in) e = hipStreamCreateWithFlags(&c->s_in, hipStreamNonBlocking);
+	if (e == hipSuccess && !c->s_out) e = hipStreamCreateWithFlags(&c->s_out, hipStreamNonBlocking);
+	for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+		hipEvent_t *evs[3] = {&c->ev_in[b], &c->ev_tr[b], &c->ev_out[b]};
+		for (hipEvent_t *ev : evs)
+			if (e == hipSuccess && !*ev) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+	}
+	if (e == hipSuccess) e = c->chunk_err.ensure(K * sizeof(uint32_t));
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: staging");
+	float *d_rays = c->rays_stage.as<float>();
+	uint8_t *d_out = c->out_stage.as<uint8_t>(), *h_out = static_cast<uint8_t *>(out);
+	for (uint64_t i = 0; i <= K; ++i) {
+		const int b = static_cast<int>(i & 1);
+		if (i < K) {
+			const uint64_t first = i * kHostChunk, cnt = std::min(kHostChunk, n - first);
+			if (i >= 2) e = hipEventSynchronize(c->ev_in[b]);  // the upload of chunk i - 2 has read pin_in[b]
+			if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray upload");
+			c->pool->copy(c->pin_in[b], rays + 6 * first, cnt * 24);
+			e = hipMemcpyAsync(d_rays + 6 * first, c->pin_in[b], cnt * 24, hipMemcpyHostToDevice, c->s_in);
+			if (e == hipSuccess) e = hipEventRecord(c->ev_in[b], c->s_in);
+			if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev_in[b], 0);
+			if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray upload");
+			if (int r = trace_common(c, d_rays + 6 * first, cnt, mask, d_out + first * stride, c->stream)) return r;
+			e = hipMemcpyAsync(c->chunk_err.as<uint32_t>() + i, c->ts.counter.as<uint32_t>() + 8, sizeof(uint32_t),
+			                   hipMemcpyDeviceToDevice, c->stream);
+			if (e == hipSuccess) e = hipEventRecord(c->ev_tr[b], c->stream);
+			if (e == hipSuccess) e = hipStreamWaitEvent(c->s_out, c->ev_tr[b], 0);
+			// pin_out[b] was emptied by the host when it copied out chunk i - 2 (previous iteration)
+			if (e == hipSuccess)
+				e = hipMemcpyAsync(c->pin_out[b], d_out + first * stride, cnt * stride, hipMemcpyDeviceToHost, c->s_out);
+			if (e == hipSuccess) e = hipEventRecord(c->ev_out[b], c->s_out);
+			if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: record download");
+		}
+		if (i >= 1) {
+			const uint64_t j = i - 1, first = j * kHostChunk, cnt = std::min(kHostChunk, n - first);
+			e = hipEventSynchronize(c->ev_out[j & 1]);
+			if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: record download");
+			c->pool->copy(h_out + first * stride, c->pin_out[j & 1], cnt * stride);
+		}
+	}
+	std::vector<uint32_t> errs(K);
+	e = hipMemcpyAsync(errs.data(), c->chunk_err.ptr, K * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: error flags");
+	uint32_t flags = 0;
+	for (uint32_t f : errs) flags |= f;
+	if (flags)
+		return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
+	return PRT_OK;
+}
+
 int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride,
                             const int32_t off[8]) {
 	if (!c) return PRT_ERR_ARG;
@@ -257,6 +346,13 @@ int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t 
 	hipError_t e = c->rays_stage.ensure(n * 24ull);
 	if (e == hipSuccess) e = c->out_stage.ensure(n * static_cast<uint64_t>(stride));
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: staging buffers");
+	if (n > kHostChunk && !c->tp.count_stats && !c->time_kernel) {
+		// large calls: pipelined through pinned buffers (statistics and kernel timing describe
+		// one launch, so those modes take the single-launch path below)
+		const int r = nearest_hits_chunked(c, rays, n, mask, out, stride);
+		if (r == PRT_OK) c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		return r;
+	}
 	e = hipMemcpyAsync(c->rays_stage.ptr, rays, n * 24ull, hipMemcpyHostToDevice, c->stream);
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray copy");
 	if (int r = trace_common(c, c->rays_stage.as<float>(), n, mask, c->out_stage.ptr, c->stream)) return r;

@@ -8,6 +8,7 @@
 
 #include "../../include/prt_mi355x.h"
 #include "batch.h"
+#include "host_copy.h"
 #include "internal.h"
 #include "mlp.h"
 
@@ -25,6 +26,14 @@ struct prt_ctx {
 	prt::TraceParams tp;
 	prt::TraceScratch ts;
 	prt::DevBuf rays_stage, out_stage;
+	// host-buffer entry point, in chunks: pinned double buffers, copy threads, and the streams
+	// that move data while the context's stream traces (created at the first large call)
+	std::unique_ptr<prt::CopyPool> pool;
+	void *pin_in[2] = {nullptr, nullptr}, *pin_out[2] = {nullptr, nullptr};
+	size_t pin_in_bytes = 0, pin_out_bytes = 0;
+	hipStream_t s_in = nullptr, s_out = nullptr;
+	hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_tr[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
+	prt::DevBuf chunk_err;
 	bool has_scene = false;
 	int time_kernel = 0;
 	int margin_log2 = 10;

@@ -184,6 +184,26 @@ def test_device_entry_point_matches_host(gpu_ctx):
     assert dev.tobytes() == host.tobytes()
 
 
+def test_host_entry_point_in_chunks(gpu_ctx):
+    """Host calls above one chunk (1,048,576 rays) are pipelined through pinned buffers: a ragged
+    count with the full record layout equals the device entry point byte for byte."""
+    import torch
+    tris = scenes.scene(2)
+    rays = scenes.rays(2, first=0, count=2_500_001)
+    gpu_ctx.set_tris(tris)
+    host = gpu_ctx.nearest_hits(rays, FULL)
+    size, _ = prt.layout(FULL)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_out = torch.zeros(len(rays) * size, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    gpu_ctx.nearest_hits_device(d_rays.data_ptr(), len(rays), FULL, d_out.data_ptr(),
+                                torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert d_out.cpu().numpy().tobytes() == host.tobytes()
+    sample = np.arange(0, len(rays), 97)
+    assert_same(host[sample], oracle.trace(tris, rays[sample]), what="chunked host path sample")
+
+
 def test_sizes_and_empty(gpu_ctx):
     tris = scenes.scene(0)[:1000]
     gpu_ctx.set_tris(tris)
