@@ -53,12 +53,12 @@ class CopyPool {
 		done_.wait(l, [this] { return pending_ == 0; });
 	}
 
-	// Threads for this process: half the CPUs it may run on, at most 8.
+	// Threads for this process: three quarters of the CPUs it may run on, at most 12.
 	static int default_threads() {
 		cpu_set_t set;
 		int cpus = 1;
 		if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = CPU_COUNT(&set);
-		return std::max(1, std::min(8, cpus / 2));
+		return std::max(1, std::min(12, cpus * 3 / 4));
 	}
 
   private:
