@@ -93,3 +93,28 @@ def test_learned_traversal_identical_to_plain(gpu_ctx, room):
     assert (got["t"][idx].view(np.uint32) == want["t"].view(np.uint32)).all()
     hit = want["t"] < np.inf
     assert (got["primitive_id"][idx][hit] == want["primitive_id"][hit]).all()
+
+
+def test_refit_requires_learned_setup_again(gpu_ctx, room):
+    """Refit changes every box, so the cluster boxes of the learned state are stale: learned
+    calls report ERR_STATE until learned_setup runs again."""
+    import torch
+    rays = scenes.rays(4, stream=2, count=4096)
+    d_rays = torch.from_numpy(rays).cuda()
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    d_out = torch.zeros(len(rays) * size, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    gpu_ctx.refit(room)
+    with pytest.raises(prt.PrtError) as e:
+        prt.nearest_hits_learned_device(gpu_ctx, d_rays.data_ptr(), len(rays), mask, d_out.data_ptr())
+    assert e.value.code == prt.ERR_STATE
+    prt.learned_setup(gpu_ctx, 7)
+    prt.nearest_hits_learned_device(gpu_ctx, d_rays.data_ptr(), len(rays), mask, d_out.data_ptr(),
+                                    torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy().view(prt.hit_dtype(mask))
+    want = oracle.trace(room, rays, oracle.LOWEST_ID)
+    assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+    hit = want["t"] < np.inf
+    assert np.array_equal(got["primitive_id"][hit], want["primitive_id"][hit])

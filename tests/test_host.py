@@ -38,7 +38,7 @@ def test_layouts_match_reference_headers():
         assert [size] + off == list(ref[mask]), (mask, size, off, ref[mask])
 
 
-def test_layout_sizes_of_survey():
+def test_layout_sizes_known_values():
     assert prt.layout(31)[0] == 32
     assert prt.layout(prt.T | prt.PRIMITIVE_ID) == (16, [-1, -1, 4, 8, -1, -1, -1, -1])
     assert prt.layout(prt.T)[0] == 16
