@@ -4,7 +4,9 @@ Runs the reference's own CPU backend (compiled from its sources into oracle/_ref
 `make -C oracle`) on:
   * the known-answer and edge-case scenes below (hand-made, tiny),
   * samples of the deterministic workloads 0, 1 (primaries and bounces) and 2 at full scene
-    size (the scenes are regenerated from their seeds; only their SHA-256 is stored).
+    size (the scenes are regenerated from their seeds; only their SHA-256 is stored),
+  * the bunny validation scene (mesh read from the reference's examples; only its hash, the
+    regenerated hit mask and a record sample are stored).
 Each fixture stores the rays (float32 x 6), the reference FullHitReg records (32 bytes each,
 t/u/v/p bit patterns, valid, and the id the reference's depth-first order picks) and the
 scene hash. Miss records keep the reference's indeterminate u/v/id bytes; tests ignore them.
@@ -88,9 +90,28 @@ def workload_samples():
     return res
 
 
+BUNNY_OBJ = "/root/reference/examples/common/bunny.obj"
+
+
+def bunny_fixture():
+    """The reference's scene-level validation (examples/validation/main.cpp:113-234): the bunny
+    mesh seen by the 1024 x 1024 camera (workload 5). Its golden image (bunny.png) is not in the
+    reference, so the hit mask is regenerated with the compiled reference backend. Only the
+    mesh hash, the mask (1 bit per ray) and a sample of full records are stored."""
+    tris = scenes.load_obj(BUNNY_OBJ)
+    rays = scenes.rays(5)
+    hits = oracle.ref_trace(tris, rays)
+    idx = np.arange(0, len(rays), 256, dtype=np.uint64)
+    np.savez_compressed(os.path.join(HERE, "bunny.npz"), mesh_sha256=np.array(scenes.sha256(tris)),
+                        n_tris=np.array(len(tris)), mask=np.packbits(hits["valid"] != 0), index=idx,
+                        hits=hits[idx.astype(np.int64)].view(np.uint8).reshape(-1, 32))
+
+
 def main():
     if not oracle.have_ref():
         raise SystemExit("build the reference shim first: make -C oracle")
+    if os.path.exists(BUNNY_OBJ):
+        bunny_fixture()
     for name, (tris, rays) in edge_scenes().items():
         hits = oracle.ref_trace(tris, rays)
         np.savez_compressed(os.path.join(HERE, f"edge_{name}.npz"), tris=tris, rays=rays,
