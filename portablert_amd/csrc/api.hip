@@ -275,10 +275,29 @@ static hipError_t ensure_pinned(void *buf[2], size_t *have, size_t want) {
 // and sent to the device on s_in while the context's stream traces chunk i - 1 and s_out
 // brings its records back; the host copies chunk i - 1's records out of their pinned buffer
 // meanwhile. Each chunk's error flags are kept (every traversal launch clears them).
+static int chunked_pass(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride);
+
 static int nearest_hits_chunked(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride) {
+	if (!c->pool) {
+		try {
+			c->pool.reset(new prt::CopyPool(prt::CopyPool::default_threads()));
+		} catch (...) {
+			return fail(c, PRT_ERR_OOM, "nearest_hits: copy threads");
+		}
+	}
+	const int r = chunked_pass(c, rays, n, mask, out, stride);
+	if (r != PRT_OK) {
+		// nothing may still read or write the pinned buffers when the next call reuses them
+		if (c->s_in) (void)hipStreamSynchronize(c->s_in);
+		if (c->s_out) (void)hipStreamSynchronize(c->s_out);
+		(void)hipStreamSynchronize(c->stream);
+	}
+	return r;
+}
+
+static int chunked_pass(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride) {
 	const uint64_t K = (n + kHostChunk - 1) / kHostChunk;
 	hipError_t e = hipSuccess;
-	if (!c->pool) c->pool.reset(new prt::CopyPool(prt::CopyPool::default_threads()));
 	e = ensure_pinned(c->pin_in, &c->pin_in_bytes, kHostChunk * 24);
 	if (e == hipSuccess) e = ensure_pinned(c->pin_out, &c->pin_out_bytes, kHostChunk * stride);
 	if (e == hipSuccess && !c->s_in) e = hipStreamCreateWithFlags(&c->s_in, hipStreamNonBlocking);

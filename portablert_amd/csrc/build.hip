@@ -1174,7 +1174,12 @@ hipError_t refit_bvh8(BuildScratch &s, Bvh8 &b, hipStream_t st, bool *bad_input,
 	if (b.levels.empty()) {
 		// level ranges from the node headers: level 0 is the root; the next level ends after the
 		// last internal child of the current one
-		std::vector<NodeF> h(b.n_nodes);
+		std::vector<NodeF> h;
+		try {
+			h.resize(b.n_nodes);
+		} catch (...) {
+			return hipErrorOutOfMemory;
+		}
 		PRT_TRY(hipMemcpyAsync(h.data(), b.nodes_f.ptr, sizeof(NodeF) * b.n_nodes, hipMemcpyDeviceToHost, st));
 		PRT_TRY(hipStreamSynchronize(st));
 		uint32_t begin = 0, end = 1;

@@ -19,7 +19,11 @@ namespace prt {
 class CopyPool {
   public:
 	explicit CopyPool(int threads) : n_(std::max(1, threads)) {
-		for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { loop(i); });
+		try {
+			for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { loop(i); });
+		} catch (...) {
+			n_ = 1 + static_cast<int>(th_.size());  // work with the threads that started
+		}
 	}
 	~CopyPool() {
 		{
