@@ -353,36 +353,38 @@ static int chunked_pass(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask
 
 int prt_mi355x_nearest_hits(prt_ctx *c, const float *rays, uint64_t n, uint32_t mask, void *out, uint32_t stride,
                             const int32_t off[8]) {
-	if (!c) return PRT_ERR_ARG;
-	c->err.clear();
-	if (int r = check_layout(c, mask, stride, off)) return r;
-	if (!c->has_scene) return fail(c, PRT_ERR_STATE, "nearest_hits called before set_tris");
-	if (n == 0) return PRT_OK;
-	if (!rays || !out) return fail(c, PRT_ERR_ARG, "nearest_hits: null pointer");
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
-	const auto t0 = std::chrono::steady_clock::now();
-	if (int r = over_limit(c, n * (24ull + stride), "nearest_hits")) return r;
-	hipError_t e = c->rays_stage.ensure(n * 24ull);
-	if (e == hipSuccess) e = c->out_stage.ensure(n * static_cast<uint64_t>(stride));
-	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: staging buffers");
-	if (n > kHostChunk && !c->tp.count_stats && !c->time_kernel) {
-		// large calls: pipelined through pinned buffers (statistics and kernel timing describe
-		// one launch, so those modes take the single-launch path below)
-		const int r = nearest_hits_chunked(c, rays, n, mask, out, stride);
-		if (r == PRT_OK) c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-		return r;
-	}
-	e = hipMemcpyAsync(c->rays_stage.ptr, rays, n * 24ull, hipMemcpyHostToDevice, c->stream);
-	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray copy");
-	if (int r = trace_common(c, c->rays_stage.as<float>(), n, mask, c->out_stage.ptr, c->stream)) return r;
-	e = hipMemcpyAsync(out, c->out_stage.ptr, n * static_cast<uint64_t>(stride), hipMemcpyDeviceToHost, c->stream);
-	uint32_t flags = 0;
-	if (e == hipSuccess) e = prt::trace_errors(c->ts, c->stream, &flags);
-	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: result copy");
-	if (flags)
-		return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
-	c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-	return PRT_OK;
+	return prt::guarded(c, [&]() -> int {
+		if (!c) return PRT_ERR_ARG;
+		c->err.clear();
+		if (int r = check_layout(c, mask, stride, off)) return r;
+		if (!c->has_scene) return fail(c, PRT_ERR_STATE, "nearest_hits called before set_tris");
+		if (n == 0) return PRT_OK;
+		if (!rays || !out) return fail(c, PRT_ERR_ARG, "nearest_hits: null pointer");
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+		const auto t0 = std::chrono::steady_clock::now();
+		if (int r = over_limit(c, n * (24ull + stride), "nearest_hits")) return r;
+		hipError_t e = c->rays_stage.ensure(n * 24ull);
+		if (e == hipSuccess) e = c->out_stage.ensure(n * static_cast<uint64_t>(stride));
+		if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: staging buffers");
+		if (n > kHostChunk && !c->tp.count_stats && !c->time_kernel) {
+			// large calls: pipelined through pinned buffers (statistics and kernel timing describe
+			// one launch, so those modes take the single-launch path below)
+			const int r = nearest_hits_chunked(c, rays, n, mask, out, stride);
+			if (r == PRT_OK) c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			return r;
+		}
+		e = hipMemcpyAsync(c->rays_stage.ptr, rays, n * 24ull, hipMemcpyHostToDevice, c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: ray copy");
+		if (int r = trace_common(c, c->rays_stage.as<float>(), n, mask, c->out_stage.ptr, c->stream)) return r;
+		e = hipMemcpyAsync(out, c->out_stage.ptr, n * static_cast<uint64_t>(stride), hipMemcpyDeviceToHost, c->stream);
+		uint32_t flags = 0;
+		if (e == hipSuccess) e = prt::trace_errors(c->ts, c->stream, &flags);
+		if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: result copy");
+		if (flags)
+			return fail(c, PRT_ERR_HIP, "nearest_hits: traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
+		c->api_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_nearest_hits_device(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t mask, void *d_out,

@@ -216,114 +216,122 @@ extern "C" {
 
 int prt_mi355x_batch_tiles(uint32_t n_scenes, const uint64_t *tri_counts, const uint64_t *ray_counts,
                            const int32_t chunk[4], uint32_t *tiles4, uint64_t *n_tiles) {
-	if (!n_tiles || (n_scenes && (!tri_counts || !ray_counts)) || !chunk) return PRT_ERR_ARG;
-	std::vector<uint64_t> to(n_scenes + 1, 0), ro(n_scenes + 1, 0);
-	for (uint32_t s = 0; s < n_scenes; ++s) {
-		to[s + 1] = to[s] + tri_counts[s];
-		ro[s + 1] = ro[s] + ray_counts[s];
-	}
-	int c[prt::kBatchBuckets];
-	for (int k = 0; k < prt::kBatchBuckets; ++k) {
-		if (chunk[k] < 1) return PRT_ERR_ARG;
-		c[k] = chunk[k];
-	}
-	const std::vector<uint32_t> t = prt::batch_plan(n_scenes, to.data(), ro.data(), c);
-	const uint64_t have = *n_tiles;
-	*n_tiles = t.size() / 4;
-	if (tiles4) {
-		if (have < t.size() / 4) return PRT_ERR_ARG;
-		std::memcpy(tiles4, t.data(), sizeof(uint32_t) * t.size());
-	}
-	return PRT_OK;
+	return prt::guarded(nullptr, [&]() -> int {
+		if (!n_tiles || (n_scenes && (!tri_counts || !ray_counts)) || !chunk) return PRT_ERR_ARG;
+		std::vector<uint64_t> to(n_scenes + 1, 0), ro(n_scenes + 1, 0);
+		for (uint32_t s = 0; s < n_scenes; ++s) {
+			to[s + 1] = to[s] + tri_counts[s];
+			ro[s + 1] = ro[s] + ray_counts[s];
+		}
+		int c[prt::kBatchBuckets];
+		for (int k = 0; k < prt::kBatchBuckets; ++k) {
+			if (chunk[k] < 1) return PRT_ERR_ARG;
+			c[k] = chunk[k];
+		}
+		const std::vector<uint32_t> t = prt::batch_plan(n_scenes, to.data(), ro.data(), c);
+		const uint64_t have = *n_tiles;
+		*n_tiles = t.size() / 4;
+		if (tiles4) {
+			if (have < t.size() / 4) return PRT_ERR_ARG;
+			std::memcpy(tiles4, t.data(), sizeof(uint32_t) * t.size());
+		}
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_batch_setup(prt_ctx *c, uint32_t n_scenes, const uint64_t *tri_counts, const uint64_t *ray_counts,
                            uint32_t mask) {
-	if (!c) return PRT_ERR_ARG;
-	if (n_scenes == 0 || n_scenes > prt::kMaxScenes || !tri_counts || !ray_counts)
-		return fail(c, PRT_ERR_ARG, "batch_setup: 1.." + std::to_string(prt::kMaxScenes) + " scenes with counts");
-	if (mask > 31u) return fail(c, PRT_ERR_ARG, "batch_setup: field mask out of range");
-	std::vector<uint64_t> to(n_scenes + 1, 0), ro(n_scenes + 1, 0);
-	for (uint32_t s = 0; s < n_scenes; ++s) {
-		to[s + 1] = to[s] + tri_counts[s];
-		ro[s + 1] = ro[s] + ray_counts[s];
-		if (to[s + 1] > static_cast<uint64_t>(prt::kTriMask) || ro[s + 1] > (1ull << 40))
-			return fail(c, PRT_ERR_ARG, "batch_setup: more than 2^29 triangles or 2^40 rays");
-	}
-	if (to.back() == 0) return fail(c, PRT_ERR_ARG, "batch_setup: the batch holds no triangle");
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_setup: device");
-	std::unique_ptr<prt::Batch> nb(new prt::Batch());
-	if (c->batch) {  // keep tuned parameters across setups
-		for (int k = 0; k < prt::kBatchBuckets; ++k) nb->bucket[k] = c->batch->bucket[k];
-		nb->block = c->batch->block;
-		nb->waves_per_cu = c->batch->waves_per_cu;
-		nb->lds_stack = c->batch->lds_stack;
-	}
-	c->batch.reset();
-	prt::Batch &b = *nb;
-	b.n_scenes = n_scenes;
-	b.tri_off = to;
-	b.ray_off = ro;
-	b.mask = mask;
-	std::vector<uint32_t> off32(to.begin(), to.end());
-	hipError_t e = b.d_tri_off.ensure(sizeof(uint32_t) * (n_scenes + 1));
-	if (e == hipSuccess)
-		e = hipMemcpyAsync(b.d_tri_off.ptr, off32.data(), sizeof(uint32_t) * (n_scenes + 1), hipMemcpyHostToDevice,
-		                   c->stream);
-	if (e == hipSuccess) e = b.d_box.ensure(sizeof(float) * 6 * n_scenes);
-	if (e == hipSuccess) e = b.d_info.ensure(sizeof(float) * 2 * n_scenes);
-	if (e == hipSuccess) e = prt::upload_params(b, c->stream);
-	if (e == hipSuccess) e = prt::upload_all_tiles(b, c->stream);
-	if (e != hipSuccess) return hip_fail(c, e, "batch_setup");
-	c->batch = std::move(nb);
-	return PRT_OK;
+	return prt::guarded(c, [&]() -> int {
+		if (!c) return PRT_ERR_ARG;
+		if (n_scenes == 0 || n_scenes > prt::kMaxScenes || !tri_counts || !ray_counts)
+			return fail(c, PRT_ERR_ARG, "batch_setup: 1.." + std::to_string(prt::kMaxScenes) + " scenes with counts");
+		if (mask > 31u) return fail(c, PRT_ERR_ARG, "batch_setup: field mask out of range");
+		std::vector<uint64_t> to(n_scenes + 1, 0), ro(n_scenes + 1, 0);
+		for (uint32_t s = 0; s < n_scenes; ++s) {
+			to[s + 1] = to[s] + tri_counts[s];
+			ro[s + 1] = ro[s] + ray_counts[s];
+			if (to[s + 1] > static_cast<uint64_t>(prt::kTriMask) || ro[s + 1] > (1ull << 40))
+				return fail(c, PRT_ERR_ARG, "batch_setup: more than 2^29 triangles or 2^40 rays");
+		}
+		if (to.back() == 0) return fail(c, PRT_ERR_ARG, "batch_setup: the batch holds no triangle");
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_setup: device");
+		std::unique_ptr<prt::Batch> nb(new prt::Batch());
+		if (c->batch) {  // keep tuned parameters across setups
+			for (int k = 0; k < prt::kBatchBuckets; ++k) nb->bucket[k] = c->batch->bucket[k];
+			nb->block = c->batch->block;
+			nb->waves_per_cu = c->batch->waves_per_cu;
+			nb->lds_stack = c->batch->lds_stack;
+		}
+		c->batch.reset();
+		prt::Batch &b = *nb;
+		b.n_scenes = n_scenes;
+		b.tri_off = to;
+		b.ray_off = ro;
+		b.mask = mask;
+		std::vector<uint32_t> off32(to.begin(), to.end());
+		hipError_t e = b.d_tri_off.ensure(sizeof(uint32_t) * (n_scenes + 1));
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(b.d_tri_off.ptr, off32.data(), sizeof(uint32_t) * (n_scenes + 1), hipMemcpyHostToDevice,
+			                   c->stream);
+		if (e == hipSuccess) e = b.d_box.ensure(sizeof(float) * 6 * n_scenes);
+		if (e == hipSuccess) e = b.d_info.ensure(sizeof(float) * 2 * n_scenes);
+		if (e == hipSuccess) e = prt::upload_params(b, c->stream);
+		if (e == hipSuccess) e = prt::upload_all_tiles(b, c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "batch_setup");
+		c->batch = std::move(nb);
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_batch_run(prt_ctx *c, const float *d_tris9, const float *d_rays6, void *d_out, void *stream) {
-	prt::Batch *b = prt::batch_of(c);
-	if (!c) return PRT_ERR_ARG;
-	if (!b) return fail(c, PRT_ERR_STATE, "batch_run before batch_setup");
-	if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_run: device");
-	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-	const bool count = c->tp.count_stats != 0;
-	hipError_t e = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, st, count ? &c->stats : nullptr);
-	if (e != hipSuccess) return hip_fail(c, e, "batch_run");
-	b->ran = true;
-	return PRT_OK;
+	return prt::guarded(c, [&]() -> int {
+		prt::Batch *b = prt::batch_of(c);
+		if (!c) return PRT_ERR_ARG;
+		if (!b) return fail(c, PRT_ERR_STATE, "batch_run before batch_setup");
+		if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_run: device");
+		hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+		const bool count = c->tp.count_stats != 0;
+		hipError_t e = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, st, count ? &c->stats : nullptr);
+		if (e != hipSuccess) return hip_fail(c, e, "batch_run");
+		b->ran = true;
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_batch_capture(prt_ctx *c, const float *d_tris9, const float *d_rays6, void *d_out) {
-	prt::Batch *b = prt::batch_of(c);
-	if (!c) return PRT_ERR_ARG;
-	if (!b) return fail(c, PRT_ERR_STATE, "batch_capture before batch_setup");
-	if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
-	if (c->tp.count_stats) return fail(c, PRT_ERR_ARG, "batch_capture: statistics counting synchronises; turn it off");
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_capture: device");
-	b->drop_graph();
-	if (!b->ran) {  // size every buffer with one eager run: nothing may allocate under capture
-		hipError_t e = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, c->stream, nullptr);
-		if (e != hipSuccess) return hip_fail(c, e, "batch_capture: sizing run");
-		if (int r = prt::batch_errors(c, *b)) return r;
-		b->ran = true;
-	}
-	hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-	if (e != hipSuccess) return hip_fail(c, e, "batch_capture: begin");
-	hipError_t ee = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, c->stream, nullptr);
-	hipGraph_t g = nullptr;
-	e = hipStreamEndCapture(c->stream, &g);
-	if (ee != hipSuccess || e != hipSuccess) {
-		if (g) (void)hipGraphDestroy(g);
-		(void)hipGetLastError();
-		return hip_fail(c, ee != hipSuccess ? ee : e, "batch_capture");
-	}
-	b->graph = g;
-	e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-	if (e != hipSuccess) {
+	return prt::guarded(c, [&]() -> int {
+		prt::Batch *b = prt::batch_of(c);
+		if (!c) return PRT_ERR_ARG;
+		if (!b) return fail(c, PRT_ERR_STATE, "batch_capture before batch_setup");
+		if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
+		if (c->tp.count_stats) return fail(c, PRT_ERR_ARG, "batch_capture: statistics counting synchronises; turn it off");
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_capture: device");
 		b->drop_graph();
-		return hip_fail(c, e, "batch_capture: instantiate");
-	}
-	return PRT_OK;
+		if (!b->ran) {  // size every buffer with one eager run: nothing may allocate under capture
+			hipError_t e = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, c->stream, nullptr);
+			if (e != hipSuccess) return hip_fail(c, e, "batch_capture: sizing run");
+			if (int r = prt::batch_errors(c, *b)) return r;
+			b->ran = true;
+		}
+		hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+		if (e != hipSuccess) return hip_fail(c, e, "batch_capture: begin");
+		hipError_t ee = prt::enqueue(c, *b, d_tris9, d_rays6, d_out, c->stream, nullptr);
+		hipGraph_t g = nullptr;
+		e = hipStreamEndCapture(c->stream, &g);
+		if (ee != hipSuccess || e != hipSuccess) {
+			if (g) (void)hipGraphDestroy(g);
+			(void)hipGetLastError();
+			return hip_fail(c, ee != hipSuccess ? ee : e, "batch_capture");
+		}
+		b->graph = g;
+		e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+		if (e != hipSuccess) {
+			b->drop_graph();
+			return hip_fail(c, e, "batch_capture: instantiate");
+		}
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_batch_replay(prt_ctx *c, void *stream) {
@@ -363,29 +371,31 @@ int prt_mi355x_batch_get_params(const prt_ctx *c, int bucket, int32_t v[3]) {
 }
 
 int prt_mi355x_batch_set_params(prt_ctx *c, int bucket, const int32_t v[3]) {
-	prt::Batch *b = prt::batch_of(c);
-	if (!c || !v) return PRT_ERR_ARG;
-	if (!b) return fail(c, PRT_ERR_STATE, "batch_set_params before batch_setup");
-	if (bucket == -1) {
-		if (!prt::valid_shape(v)) return fail(c, PRT_ERR_ARG, "batch_set_params: block 64/128/256, waves 1..64, stack 8/16");
-		b->block = v[0];
-		b->waves_per_cu = v[1];
-		b->lds_stack = v[2];
-	} else if (bucket >= 0 && bucket < prt::kBatchBuckets) {
-		if (!prt::valid_bucket(v)) return fail(c, PRT_ERR_ARG, "batch_set_params: width 2..8, leaf 1..4, chunk 1..2^20");
-		b->bucket[bucket].width = v[0];
-		b->bucket[bucket].leaf = v[1];
-		b->bucket[bucket].chunk = v[2];
-	} else {
-		return fail(c, PRT_ERR_ARG, "batch_set_params: bucket -1..3");
-	}
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_set_params: device");
-	hipError_t e = prt::upload_params(*b, c->stream);
-	if (e == hipSuccess) e = prt::upload_all_tiles(*b, c->stream);
-	if (e != hipSuccess) return hip_fail(c, e, "batch_set_params");
-	b->drop_graph();
-	b->ran = false;
-	return PRT_OK;
+	return prt::guarded(c, [&]() -> int {
+		prt::Batch *b = prt::batch_of(c);
+		if (!c || !v) return PRT_ERR_ARG;
+		if (!b) return fail(c, PRT_ERR_STATE, "batch_set_params before batch_setup");
+		if (bucket == -1) {
+			if (!prt::valid_shape(v)) return fail(c, PRT_ERR_ARG, "batch_set_params: block 64/128/256, waves 1..64, stack 8/16");
+			b->block = v[0];
+			b->waves_per_cu = v[1];
+			b->lds_stack = v[2];
+		} else if (bucket >= 0 && bucket < prt::kBatchBuckets) {
+			if (!prt::valid_bucket(v)) return fail(c, PRT_ERR_ARG, "batch_set_params: width 2..8, leaf 1..4, chunk 1..2^20");
+			b->bucket[bucket].width = v[0];
+			b->bucket[bucket].leaf = v[1];
+			b->bucket[bucket].chunk = v[2];
+		} else {
+			return fail(c, PRT_ERR_ARG, "batch_set_params: bucket -1..3");
+		}
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_set_params: device");
+		hipError_t e = prt::upload_params(*b, c->stream);
+		if (e == hipSuccess) e = prt::upload_all_tiles(*b, c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "batch_set_params");
+		b->drop_graph();
+		b->ran = false;
+		return PRT_OK;
+	});
 }
 
 // Autotuner. Stage 1: every (width, leaf) candidate is applied to all buckets at once, the batch
@@ -394,124 +404,126 @@ int prt_mi355x_batch_set_params(prt_ctx *c, int bucket, const int32_t v[3]) {
 // parameters are kept only if build + traversal of the whole batch got faster.
 int prt_mi355x_batch_tune(prt_ctx *c, const float *d_tris9, const float *d_rays6, void *d_out, int reps,
                           double *before_ms, double *after_ms) {
-	prt::Batch *b = prt::batch_of(c);
-	if (!c) return PRT_ERR_ARG;
-	if (!b) return fail(c, PRT_ERR_STATE, "batch_tune before batch_setup");
-	if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_tune: device");
-	reps = std::max(1, std::min(16, reps));
-	hipStream_t st = c->stream;
-	const int saved_count = c->tp.count_stats;
-	c->tp.count_stats = 0;
-	b->drop_graph();
-	prt::Batch &B = *b;
-	const prt::BucketParams orig[prt::kBatchBuckets] = {B.bucket[0], B.bucket[1], B.bucket[2], B.bucket[3]};
-	const int orig_shape[3] = {B.block, B.waves_per_cu, B.lds_stack};
-	hipError_t e = hipSuccess;
-	auto whole = [&]() { return prt::enqueue(c, B, d_tris9, d_rays6, d_out, st, nullptr); };
-	double t_before = 0.0, t_after = 0.0;
-	// one untimed run sizes the buffers
-	if (e == hipSuccess) e = whole();
-	if (e == hipSuccess) e = hipStreamSynchronize(st);
-	if (e == hipSuccess) e = prt::time_min(st, reps, whole, &t_before);
-	// per-bucket subsets of tiles
-	auto bucket_time = [&](int bk, const int chunk[prt::kBatchBuckets], double *ms) -> hipError_t {
-		const std::vector<uint32_t> t = prt::batch_plan(B.n_scenes, B.tri_off.data(), B.ray_off.data(), chunk, bk);
-		*ms = -1.0;
-		if (t.empty()) return hipSuccess;
-		uint64_t n = 0;
-		for (size_t i = 2; i < t.size(); i += 4) n += t[i];
-		hipError_t r = prt::upload_tiles(t, B.d_tiles_sub, st);
-		if (r != hipSuccess) return r;
-		const uint32_t nt = static_cast<uint32_t>(t.size() / 4);
-		return prt::time_min(st, reps, [&]() { return prt::enqueue_trace(c, B, d_rays6, d_out, st, B.d_tiles_sub, nt, n, nullptr); }, ms);
-	};
-	int cur_chunk[prt::kBatchBuckets];
-	for (int k = 0; k < prt::kBatchBuckets; ++k) cur_chunk[k] = B.bucket[k].chunk;
-	// stage 1: branching factor and leaf size
-	const int cand[][2] = {{8, 1}, {8, 2}, {8, 3}, {8, 4}, {4, 1}, {4, 2}, {4, 4}, {6, 2}};
-	double best[prt::kBatchBuckets];
-	prt::BucketParams pick[prt::kBatchBuckets];
-	for (int k = 0; k < prt::kBatchBuckets; ++k) {
-		best[k] = 1e30;
-		pick[k] = B.bucket[k];
-	}
-	for (const auto &cd : cand) {
-		if (e != hipSuccess) break;
+	return prt::guarded(c, [&]() -> int {
+		prt::Batch *b = prt::batch_of(c);
+		if (!c) return PRT_ERR_ARG;
+		if (!b) return fail(c, PRT_ERR_STATE, "batch_tune before batch_setup");
+		if (int r = prt::check_io(c, d_tris9, d_rays6, d_out)) return r;
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_NODEV, "batch_tune: device");
+		reps = std::max(1, std::min(16, reps));
+		hipStream_t st = c->stream;
+		const int saved_count = c->tp.count_stats;
+		c->tp.count_stats = 0;
+		b->drop_graph();
+		prt::Batch &B = *b;
+		const prt::BucketParams orig[prt::kBatchBuckets] = {B.bucket[0], B.bucket[1], B.bucket[2], B.bucket[3]};
+		const int orig_shape[3] = {B.block, B.waves_per_cu, B.lds_stack};
+		hipError_t e = hipSuccess;
+		auto whole = [&]() { return prt::enqueue(c, B, d_tris9, d_rays6, d_out, st, nullptr); };
+		double t_before = 0.0, t_after = 0.0;
+		// one untimed run sizes the buffers
+		if (e == hipSuccess) e = whole();
+		if (e == hipSuccess) e = hipStreamSynchronize(st);
+		if (e == hipSuccess) e = prt::time_min(st, reps, whole, &t_before);
+		// per-bucket subsets of tiles
+		auto bucket_time = [&](int bk, const int chunk[prt::kBatchBuckets], double *ms) -> hipError_t {
+			const std::vector<uint32_t> t = prt::batch_plan(B.n_scenes, B.tri_off.data(), B.ray_off.data(), chunk, bk);
+			*ms = -1.0;
+			if (t.empty()) return hipSuccess;
+			uint64_t n = 0;
+			for (size_t i = 2; i < t.size(); i += 4) n += t[i];
+			hipError_t r = prt::upload_tiles(t, B.d_tiles_sub, st);
+			if (r != hipSuccess) return r;
+			const uint32_t nt = static_cast<uint32_t>(t.size() / 4);
+			return prt::time_min(st, reps, [&]() { return prt::enqueue_trace(c, B, d_rays6, d_out, st, B.d_tiles_sub, nt, n, nullptr); }, ms);
+		};
+		int cur_chunk[prt::kBatchBuckets];
+		for (int k = 0; k < prt::kBatchBuckets; ++k) cur_chunk[k] = B.bucket[k].chunk;
+		// stage 1: branching factor and leaf size
+		const int cand[][2] = {{8, 1}, {8, 2}, {8, 3}, {8, 4}, {4, 1}, {4, 2}, {4, 4}, {6, 2}};
+		double best[prt::kBatchBuckets];
+		prt::BucketParams pick[prt::kBatchBuckets];
 		for (int k = 0; k < prt::kBatchBuckets; ++k) {
-			B.bucket[k].width = cd[0];
-			B.bucket[k].leaf = cd[1];
+			best[k] = 1e30;
+			pick[k] = B.bucket[k];
 		}
-		e = prt::upload_params(B, st);
-		if (e == hipSuccess) e = prt::enqueue_build(c, B, d_tris9, st);
-		for (int k = 0; k < prt::kBatchBuckets && e == hipSuccess; ++k) {
-			double ms = 0.0;
-			e = bucket_time(k, cur_chunk, &ms);
-			if (e == hipSuccess && ms >= 0.0 && ms < best[k]) {
-				best[k] = ms;
-				pick[k].width = cd[0];
-				pick[k].leaf = cd[1];
+		for (const auto &cd : cand) {
+			if (e != hipSuccess) break;
+			for (int k = 0; k < prt::kBatchBuckets; ++k) {
+				B.bucket[k].width = cd[0];
+				B.bucket[k].leaf = cd[1];
 			}
-		}
-	}
-	for (int k = 0; k < prt::kBatchBuckets; ++k) B.bucket[k] = pick[k];
-	// stage 2: tile size
-	if (e == hipSuccess) e = prt::upload_params(B, st);
-	if (e == hipSuccess) e = prt::enqueue_build(c, B, d_tris9, st);
-	const int chunks[] = {64, 128, 256, 512};
-	for (int k = 0; k < prt::kBatchBuckets; ++k) best[k] = 1e30;
-	for (int ch : chunks) {
-		int cc[prt::kBatchBuckets] = {ch, ch, ch, ch};
-		for (int k = 0; k < prt::kBatchBuckets && e == hipSuccess; ++k) {
-			double ms = 0.0;
-			e = bucket_time(k, cc, &ms);
-			if (e == hipSuccess && ms >= 0.0 && ms < best[k]) {
-				best[k] = ms;
-				B.bucket[k].chunk = ch;
-			}
-		}
-	}
-	// stage 3: launch shape over the whole batch
-	if (e == hipSuccess) e = prt::upload_all_tiles(B, st);
-	double best_shape = 1e30;
-	int shape[3] = {B.block, B.waves_per_cu, B.lds_stack};
-	const int blocks[] = {64, 128, 256}, stacks[] = {8, 16}, waves[] = {20, 24, 32};
-	for (int bl : blocks)
-		for (int sk : stacks)
-			for (int wv : waves) {
-				if (e != hipSuccess) break;
-				B.block = bl;
-				B.lds_stack = sk;
-				B.waves_per_cu = wv;
+			e = prt::upload_params(B, st);
+			if (e == hipSuccess) e = prt::enqueue_build(c, B, d_tris9, st);
+			for (int k = 0; k < prt::kBatchBuckets && e == hipSuccess; ++k) {
 				double ms = 0.0;
-				e = prt::time_min(st, reps, [&]() { return prt::enqueue_trace(c, B, d_rays6, d_out, st, B.d_tiles, B.n_tiles, B.ray_off.back(), nullptr); }, &ms);
-				if (e == hipSuccess && ms < best_shape) {
-					best_shape = ms;
-					shape[0] = bl;
-					shape[1] = wv;
-					shape[2] = sk;
+				e = bucket_time(k, cur_chunk, &ms);
+				if (e == hipSuccess && ms >= 0.0 && ms < best[k]) {
+					best[k] = ms;
+					pick[k].width = cd[0];
+					pick[k].leaf = cd[1];
 				}
 			}
-	B.block = shape[0];
-	B.waves_per_cu = shape[1];
-	B.lds_stack = shape[2];
-	if (e == hipSuccess) e = prt::time_min(st, reps, whole, &t_after);
-	if (e == hipSuccess && t_after > t_before) {  // keep what was there
-		for (int k = 0; k < prt::kBatchBuckets; ++k) B.bucket[k] = orig[k];
-		B.block = orig_shape[0];
-		B.waves_per_cu = orig_shape[1];
-		B.lds_stack = orig_shape[2];
-		e = prt::upload_params(B, st);
+		}
+		for (int k = 0; k < prt::kBatchBuckets; ++k) B.bucket[k] = pick[k];
+		// stage 2: tile size
+		if (e == hipSuccess) e = prt::upload_params(B, st);
+		if (e == hipSuccess) e = prt::enqueue_build(c, B, d_tris9, st);
+		const int chunks[] = {64, 128, 256, 512};
+		for (int k = 0; k < prt::kBatchBuckets; ++k) best[k] = 1e30;
+		for (int ch : chunks) {
+			int cc[prt::kBatchBuckets] = {ch, ch, ch, ch};
+			for (int k = 0; k < prt::kBatchBuckets && e == hipSuccess; ++k) {
+				double ms = 0.0;
+				e = bucket_time(k, cc, &ms);
+				if (e == hipSuccess && ms >= 0.0 && ms < best[k]) {
+					best[k] = ms;
+					B.bucket[k].chunk = ch;
+				}
+			}
+		}
+		// stage 3: launch shape over the whole batch
 		if (e == hipSuccess) e = prt::upload_all_tiles(B, st);
+		double best_shape = 1e30;
+		int shape[3] = {B.block, B.waves_per_cu, B.lds_stack};
+		const int blocks[] = {64, 128, 256}, stacks[] = {8, 16}, waves[] = {20, 24, 32};
+		for (int bl : blocks)
+			for (int sk : stacks)
+				for (int wv : waves) {
+					if (e != hipSuccess) break;
+					B.block = bl;
+					B.lds_stack = sk;
+					B.waves_per_cu = wv;
+					double ms = 0.0;
+					e = prt::time_min(st, reps, [&]() { return prt::enqueue_trace(c, B, d_rays6, d_out, st, B.d_tiles, B.n_tiles, B.ray_off.back(), nullptr); }, &ms);
+					if (e == hipSuccess && ms < best_shape) {
+						best_shape = ms;
+						shape[0] = bl;
+						shape[1] = wv;
+						shape[2] = sk;
+					}
+				}
+		B.block = shape[0];
+		B.waves_per_cu = shape[1];
+		B.lds_stack = shape[2];
 		if (e == hipSuccess) e = prt::time_min(st, reps, whole, &t_after);
-	}
-	c->tp.count_stats = saved_count;
-	if (e != hipSuccess) return hip_fail(c, e, "batch_tune");
-	B.ran = true;
-	if (int r = prt::batch_errors(c, B)) return r;
-	if (before_ms) *before_ms = t_before;
-	if (after_ms) *after_ms = t_after;
-	return PRT_OK;
+		if (e == hipSuccess && t_after > t_before) {  // keep what was there
+			for (int k = 0; k < prt::kBatchBuckets; ++k) B.bucket[k] = orig[k];
+			B.block = orig_shape[0];
+			B.waves_per_cu = orig_shape[1];
+			B.lds_stack = orig_shape[2];
+			e = prt::upload_params(B, st);
+			if (e == hipSuccess) e = prt::upload_all_tiles(B, st);
+			if (e == hipSuccess) e = prt::time_min(st, reps, whole, &t_after);
+		}
+		c->tp.count_stats = saved_count;
+		if (e != hipSuccess) return hip_fail(c, e, "batch_tune");
+		B.ran = true;
+		if (int r = prt::batch_errors(c, B)) return r;
+		if (before_ms) *before_ms = t_before;
+		if (after_ms) *after_ms = t_after;
+		return PRT_OK;
+	});
 }
 
 }  // extern "C"

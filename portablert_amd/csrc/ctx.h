@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <new>
 #include <string>
 
 #include "../../include/prt_mi355x.h"
@@ -71,6 +72,19 @@ inline hipError_t wait_use(prt_ctx *c) {
 	if (!c->busy_set) return hipSuccess;
 	c->busy_set = false;
 	return hipStreamWaitEvent(c->stream, c->busy, 0);
+}
+
+// Runs an entry point's body; a failed host allocation (or any other C++ exception) becomes an
+// error code instead of crossing the C ABI.
+template <class F>
+inline int guarded(prt_ctx *c, F &&body) {
+	try {
+		return body();
+	} catch (const std::bad_alloc &) {
+		return fail(c, PRT_ERR_OOM, "host memory allocation failed");
+	} catch (...) {
+		return fail(c, PRT_ERR_HIP, "unexpected host error");
+	}
 }
 
 inline int hip_fail(prt_ctx *c, hipError_t e, const char *what) {

@@ -209,23 +209,25 @@ using namespace prt;
 extern "C" {
 
 int prt_mi355x_learned_setup(prt_ctx *c, uint64_t seed) {
-	if (!c) return PRT_ERR_ARG;
-	c->err.clear();
-	if (!c->has_scene) return fail(c, PRT_ERR_STATE, "learned_setup: set_tris first");
-	if (c->bvh.n_tris == 0) return fail(c, PRT_ERR_STATE, "learned_setup: empty scene");
-	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
-	hipError_t e = build_clusters(c);
-	if (e != hipSuccess) return hip_fail(c, e, "learned_setup: clusters");
-	for (int a = 0; a < 3; ++a) {
-		const float lo = c->bvh.scene_box[a], hi = c->bvh.scene_box[3 + a];
-		c->mlp.scene_lo[a] = lo;
-		c->mlp.scene_inv_ext[a] = hi > lo ? 1.0f / (hi - lo) : 1.0f;
-	}
-	e = mlp_init(c->mlp, seed, c->stream);
-	if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-	if (e != hipSuccess) return hip_fail(c, e, "learned_setup: network");
-	c->learned_scene = c->scene_gen;
-	return PRT_OK;
+	return prt::guarded(c, [&]() -> int {
+		if (!c) return PRT_ERR_ARG;
+		c->err.clear();
+		if (!c->has_scene) return fail(c, PRT_ERR_STATE, "learned_setup: set_tris first");
+		if (c->bvh.n_tris == 0) return fail(c, PRT_ERR_STATE, "learned_setup: empty scene");
+		if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
+		hipError_t e = build_clusters(c);
+		if (e != hipSuccess) return hip_fail(c, e, "learned_setup: clusters");
+		for (int a = 0; a < 3; ++a) {
+			const float lo = c->bvh.scene_box[a], hi = c->bvh.scene_box[3 + a];
+			c->mlp.scene_lo[a] = lo;
+			c->mlp.scene_inv_ext[a] = hi > lo ? 1.0f / (hi - lo) : 1.0f;
+		}
+		e = mlp_init(c->mlp, seed, c->stream);
+		if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+		if (e != hipSuccess) return hip_fail(c, e, "learned_setup: network");
+		c->learned_scene = c->scene_gen;
+		return PRT_OK;
+	});
 }
 
 int prt_mi355x_learned_train(prt_ctx *c, const float *d_rays, uint64_t n, int epochs, int batch, float lr) {
