@@ -129,7 +129,11 @@ class Context:
             lib().prt_mi355x_destroy(self.h)
             self.h = None
 
-    __del__ = close
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: module globals may already be gone
+            pass
 
     def _check(self, rc):
         if rc:
