@@ -528,8 +528,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *__restrict__
 		const float dx = fmaxf(b.x, e.x) - fminf(a.x, c.x);
 		const float dy = fmaxf(b.y, e.y) - fminf(a.y, c.y);
 		const float dz = fmaxf(b.z, e.z) - fminf(a.z, c.z);
-		const float ar = dx * dy + dx * dz + dy * dz;
-		bool take = ar < best;
+		const float ar = dx * dy + dx * dz + dy * dz;  // +inf for huge coordinates: ties then decide
+		bool take = bj < 0 || ar < best;
 		if (ar == best && bj >= 0) {
 			const int lo = min(i, j), hi = max(i, j), blo = min(i, bj), bhi = max(i, bj);
 			take = lo < blo || (lo == blo && hi < bhi);

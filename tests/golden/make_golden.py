@@ -66,6 +66,28 @@ def edge_scenes():
         d = rng.normal(size=(256, 3))
         d /= np.linalg.norm(d, axis=1, keepdims=True)
         out[f"soup{n}"] = (tris, np.concatenate([o, d], 1).astype(np.float32))
+    # degenerate triangles among ordinary ones: zero area (collinear), a point, exact duplicates
+    # (equal t on different ids) and triangles sharing an edge
+    base = (rng.uniform(-1, 1, (40, 1, 3)) + rng.uniform(-0.3, 0.3, (40, 3, 3))).reshape(40, 9)
+    col = np.array([[-1, -1, 0, 0, 0, 0, 1, 1, 0], [0.2, 0.2, 0.2] * 3, [-1, 0, 0.3, 1, 0, 0.3, 0, 0, 0.3]])
+    dup = np.concatenate([base[:5], base[:5], col, base[5:]]).astype(np.float32)
+    o = rng.uniform(-1.2, 1.2, (256, 3))
+    d = rng.normal(size=(256, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out["degenerate"] = (dup, np.concatenate([o, d], 1).astype(np.float32))
+    # non-finite and degenerate rays: NaN or inf in origin or direction, a zero direction
+    nf = np.array([[np.nan, 0, -1, 0, 0, 1], [0, np.nan, -1, 0, 0, 1], [0.1, 0, -1, np.nan, 0, 1],
+                   [0.1, 0, -1, 0, 0, np.nan], [np.inf, 0, -1, 0, 0, 1], [0.1, 0, -np.inf, 0, 0, 1],
+                   [0.1, 0, -1, np.inf, 0, 1], [0.1, 0, -1, 0, 0, np.inf], [0.1, 0, -1, 0, 0, -np.inf],
+                   [0.1, 0, -1, 0, 0, 0], [0, 0, 0, 0, 0, 0], [0.1, 0, -1, 0, 0, 1e-40]], np.float32)
+    out["nonfinite_rays"] = (np.concatenate([kt, dup]), nf)
+    # the soups scaled far up and far down: coordinates beyond 2^60 (no fast node bounds) and
+    # near the bottom of the normal range
+    s200, r200 = out["soup200"]
+    for name, k in (("huge", 1e19), ("tiny", 1e-19)):
+        r = r200.copy()
+        r[:, :3] *= np.float32(k)
+        out[name] = ((s200 * np.float32(k)).astype(np.float32), r)
     return out
 
 

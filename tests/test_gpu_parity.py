@@ -144,6 +144,29 @@ def test_scheduling_options(gpu_ctx, opts):
     assert_same(got, want, fields=("t", "primitive_id", "u", "v"), what=str(opts))
 
 
+def test_ray_order_does_not_matter(gpu_ctx):
+    """All 8,388,608 room rays in a random order give the same record for every ray (a full-size
+    property: which wave takes a ray and what runs beside it change nothing)."""
+    import torch
+    tris, rays = scenes.scene(2), scenes.rays(2)
+    mask = prt.T | prt.PRIMITIVE_ID | prt.UV
+    size, _ = prt.layout(mask)
+    gpu_ctx.set_tris(tris)
+    perm = np.random.default_rng(5).permutation(len(rays))
+    d_rays = torch.from_numpy(rays).cuda()
+    d_perm = torch.from_numpy(np.ascontiguousarray(rays[perm])).cuda()
+    out_a = torch.empty(len(rays) * size, dtype=torch.uint8, device="cuda")
+    out_b = torch.empty_like(out_a)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream().cuda_stream
+    gpu_ctx.nearest_hits_device(d_rays.data_ptr(), len(rays), mask, out_a.data_ptr(), st)
+    gpu_ctx.nearest_hits_device(d_perm.data_ptr(), len(rays), mask, out_b.data_ptr(), st)
+    torch.cuda.synchronize()
+    a = out_a.cpu().numpy().reshape(len(rays), size)
+    b = out_b.cpu().numpy().reshape(len(rays), size)
+    assert np.array_equal(a[perm], b)
+
+
 def test_all_field_masks(gpu_ctx):
     """Every HitReg<Tags...> layout holds the same values as the full record."""
     tris = scenes.scene(0)
