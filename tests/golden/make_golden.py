@@ -82,12 +82,17 @@ def edge_scenes():
                    [0.1, 0, -1, 0, 0, 0], [0, 0, 0, 0, 0, 0], [0.1, 0, -1, 0, 0, 1e-40]], np.float32)
     out["nonfinite_rays"] = (np.concatenate([kt, dup]), nf)
     # the soups scaled far up and far down: coordinates beyond 2^60 (no fast node bounds) and
-    # near the bottom of the normal range
+    # near the bottom of the normal range. At 1e19 Moller-Trumbore overflows for most rays and at
+    # 1e-19 1/det does for all of them, so the same soups scaled by 1e12 and 1e-12 (which still
+    # hit) and the soup with one far triangle at 2^61 (a scene box beyond 2^60) join them
     s200, r200 = out["soup200"]
-    for name, k in (("huge", 1e19), ("tiny", 1e-19)):
+    for name, k in (("huge", 1e19), ("tiny", 1e-19), ("scaled_up", 1e12), ("scaled_down", 1e-12)):
         r = r200.copy()
         r[:, :3] *= np.float32(k)
         out[name] = ((s200 * np.float32(k)).astype(np.float32), r)
+    far = np.float32(2.0 ** 61)
+    ft = np.array([[far, far, far, 1.5 * far, far, far, far, 1.5 * far, far]], np.float32)
+    out["far_triangle"] = (np.concatenate([s200, ft]), r200.copy())
     return out
 
 

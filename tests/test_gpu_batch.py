@@ -101,6 +101,35 @@ def test_mixed_scenes_vs_oracle(gpu_ctx):
     assert (dev.records(d_out, 1)["valid"] == 0).all()
 
 
+def test_extreme_scales_in_one_batch(gpu_ctx):
+    """Scenes of very different scales built and traced together: one scaled by 1e12, one by
+    1e-12, and an ordinary one with a single far triangle at 2^61 (its box is beyond 2^60, so its
+    nodes take the exact two-step test instead of the one-fma form)."""
+    rng = np.random.default_rng(29)
+    base = [soup(rng, n) for n in (500, 2000, 500)]
+    rays = [ray_set(rng, n) for n in (600, 600, 600)]
+    far = np.float32(2.0 ** 61)
+    tris_list, rays_list = [], []
+    for t, r, k in zip(base, rays, (1e12, 1.0, 1e-12)):
+        rr = r.copy()
+        rr[:, :3] *= np.float32(k)
+        tt = (t * np.float32(k)).astype(np.float32)
+        if k == 1.0:
+            tt = np.concatenate([tt, np.array([[far, far, far, 1.5 * far, far, far, far, 1.5 * far, far]],
+                                              np.float32)])
+        tris_list.append(tt)
+        rays_list.append(rr)
+    dev = Device(tris_list, rays_list)
+    print("extreme batch:", [len(t) for t in tris_list], flush=True)
+    batch = prt.Batch(gpu_ctx, dev.tc, dev.rc, FULL)
+    d_out = run_eager(gpu_ctx, dev, batch)
+    for s, (t, r) in enumerate(zip(tris_list, rays_list)):
+        want = oracle.trace(t, r, oracle.LOWEST_ID)
+        got = dev.records(d_out, s)
+        assert_same(got, want, what=f"scene {s}")
+        assert (got["valid"] == 1).sum() > 10
+
+
 def test_non_finite_vertex_reported(gpu_ctx):
     rng = np.random.default_rng(3)
     t0, t1 = soup(rng, 50), soup(rng, 60)
