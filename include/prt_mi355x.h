@@ -51,7 +51,9 @@ enum {
 	PRT_OPT_MARGIN_REL_LOG2 = 12, /* culling margin relative part 2^-k, k in 4..30 (default 10) */
 	PRT_OPT_BUILDER = 13,     /* binary tree under the BVH8: 0 radix tree over the Morton order, 1
 	                             clustering of Morton-order neighbours (default); at set_tris */
-	PRT_OPT_REFILL = 14       /* a wave loads new rays once at least v of its lanes are idle, 1..64 (default 8) */
+	PRT_OPT_REFILL = 14,      /* a wave loads new rays once at least v of its lanes are idle, 1..64 (default 8) */
+	PRT_OPT_COLLAPSE = 15     /* binary tree to BVH8: 0 greedy (open the largest child until full), 1
+	                             least surface-area-heuristic cost (default); at set_tris */
 };
 
 /* Statistics (prt_mi355x_get_stat). */

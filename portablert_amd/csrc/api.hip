@@ -467,6 +467,10 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		if (v != 0 && v != 1) return bad();
 		c->bp.builder = static_cast<int>(v);
 		break;
+	case PRT_OPT_COLLAPSE:
+		if (v != 0 && v != 1) return bad();
+		c->bp.collapse = static_cast<int>(v);
+		break;
 	default: return fail(c, PRT_ERR_ARG, "unknown option " + std::to_string(opt));
 	}
 	return PRT_OK;
@@ -489,6 +493,7 @@ int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
 	case PRT_OPT_ALLOC_LIMIT: *v = c->alloc_limit; break;
 	case PRT_OPT_MARGIN_REL_LOG2: *v = c->margin_log2; break;
 	case PRT_OPT_BUILDER: *v = c->bp.builder; break;
+	case PRT_OPT_COLLAPSE: *v = c->bp.collapse; break;
 	default: return PRT_ERR_ARG;
 	}
 	return PRT_OK;

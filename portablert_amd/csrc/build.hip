@@ -494,6 +494,11 @@ __global__ __launch_bounds__(kBlock) void k_ploc_init(const float4 *__restrict__
 	t.box[2ull * node + 1] = tbox[2ull * tri + 1];
 	t.count[node] = 1u;
 	t.first[node] = j;
+	t.parent[node] = kNone;  // scene roots keep kNone; merges set the others
+	if (j + 1 < t.n) {
+		t.parent[j] = kNone;
+		t.flag[j] = 0u;
+	}
 	clu[j] = node;
 }
 
@@ -566,6 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restric
 				t.box[2ull * id + 1] = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z), 0.0f);
 				t.left[id] = l;
 				t.right[id] = r;
+				t.parent[l] = id;
+				t.parent[r] = id;
 				t.count[id] = t.count[l] + t.count[r];
 				t.first[id] = t.first[l];
 				node = id;
@@ -651,6 +658,117 @@ __device__ __forceinline__ int grid_exp(float p, float top) {
 	return e;
 }
 
+// Leaf size and node width of the scene of binary node b (per-scene parameters when given).
+__device__ __forceinline__ void node_shape(const BinTree &t, const uint64_t *keys, const uint32_t *param, uint32_t b,
+                                           int leaf_size, int width, int &ls, int &w) {
+	if (param) {
+		const uint32_t pr = param[keys[t.first[b]] >> kSceneShift];
+		leaf_size = static_cast<int>(pr & 0xFFu);
+		width = static_cast<int>((pr >> 8) & 0xFFu);
+	}
+	ls = max(1, min(kMaxLeaf, leaf_size));
+	w = max(2, min(8, width));
+}
+
+// ------------------------------------------- collapse costs (surface-area heuristic, SAH)
+//
+// Cost-optimal collapse of the binary tree into nodes of at most `width` children, by the
+// dynamic program of Ylitie, Karras and Laine 2017 ("Efficient incoherent ray traversal on GPUs
+// through compressed wide BVHs", section 3.1). For binary node x and i in 1..7 slots, C(x, i) is
+// the least expected cost of x's subtree when it may fill i child slots of the node above it:
+//   C(x, 1) = min(leaf(x), A(x) * kCostNode + split(x, width))
+//   C(x, i) = min(C(x, i - 1), split(x, i))                               for i >= 2
+//   split(x, i) = min over k in 1..i-1 of C(left(x), k) + C(right(x), i - k)
+//   leaf(x) = A(x) * kCostTri * P(x) if x holds at most leaf_size triangles, else +inf
+// with A the half surface area of x's box and P its triangle count; a single triangle costs
+// A * kCostTri in any number of slots. The greedy alternative (open the largest child until the
+// node is full) is kept as collapse mode 0. Costs are doubles: fp32 areas overflow for
+// coordinates near 1e19 and vanish near 1e-19.
+constexpr double kCostNode = 1.0;
+constexpr double kCostTri = 0.3;
+constexpr int kCostRow = 8;  // C(x, 1..7), then 1.0 if x is best kept as one leaf, else 0.0
+
+__device__ __forceinline__ double dhalf_area(float4 lo, float4 hi) {
+	const double dx = static_cast<double>(hi.x) - lo.x, dy = static_cast<double>(hi.y) - lo.y,
+	             dz = static_cast<double>(hi.z) - lo.z;
+	return dx * dy + dx * dz + dy * dz;
+}
+
+// split(x, i) from the rows of x's children, and the k that attains it (the smallest on ties).
+__device__ __forceinline__ double best_split(const double *cl, const double *cr, int i, int &kbest) {
+	double best = cl[0] + cr[i - 2];
+	kbest = 1;
+	for (int k = 2; k < i; ++k) {
+		const double v = cl[k - 1] + cr[i - k - 1];
+		if (v < best) {
+			best = v;
+			kbest = k;
+		}
+	}
+	return best;
+}
+
+// Rows written by one thread and read by a thread of another workgroup in the same launch:
+// device-scope atomic 8-byte stores and loads, as for the boxes.
+__device__ __forceinline__ void st_row(double *cost, uint32_t x, const double c[kCostRow]) {
+	uint64_t *p = reinterpret_cast<uint64_t *>(cost + static_cast<uint64_t>(kCostRow) * x);
+#pragma unroll
+	for (int k = 0; k < kCostRow; ++k)
+		__hip_atomic_store(p + k, static_cast<uint64_t>(__double_as_longlong(c[k])), __ATOMIC_RELAXED,
+		                   __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ld_row(const double *cost, uint32_t x, double c[kCostRow]) {
+	const uint64_t *p = reinterpret_cast<const uint64_t *>(cost + static_cast<uint64_t>(kCostRow) * x);
+#pragma unroll
+	for (int k = 0; k < kCostRow; ++k)
+		c[k] = __longlong_as_double(static_cast<long long>(__hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+}
+
+struct CostArgs {
+	BinTree t;
+	double *cost;  // kCostRow doubles per binary node
+	const uint64_t *keys;
+	const uint32_t *param;
+	int leaf_size;
+	int width;
+};
+
+// Rows leaf to root: every triangle starts at its leaf, the second child to finish computes the
+// parent (flags zero on entry, parents kNone at the roots).
+__global__ __launch_bounds__(kBlock) void k_sah_cost(CostArgs a) {
+	const BinTree &t = a.t;
+	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	if (j >= t.n) return;
+	const uint32_t x = t.n - 1 + j;
+	double c[kCostRow];
+	const double leaf = dhalf_area(t.box[2ull * x], t.box[2ull * x + 1]) * kCostTri;
+#pragma unroll
+	for (int k = 0; k < kCostRow - 1; ++k) c[k] = leaf;
+	c[kCostRow - 1] = 1.0;
+	st_row(a.cost, x, c);
+	uint32_t p = t.n > 1 ? t.parent[x] : kNone;
+	for (uint32_t guard = 0; p != kNone && guard < t.n; ++guard) {
+		stores_done();
+		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
+		double cl[kCostRow], cr[kCostRow];
+		ld_row(a.cost, t.left[p], cl);
+		ld_row(a.cost, t.right[p], cr);
+		int ls, w, kb;
+		node_shape(t, a.keys, a.param, p, a.leaf_size, a.width, ls, w);
+		const double area = dhalf_area(t.box[2ull * p], t.box[2ull * p + 1]);
+		const uint32_t count = t.count[p];
+		const bool can_leaf = count <= static_cast<uint32_t>(ls);
+		const double as_leaf = can_leaf ? area * kCostTri * count : __builtin_inf();
+		const double as_node = area * kCostNode + best_split(cl, cr, w, kb);
+		const bool leaf_best = can_leaf && as_leaf <= as_node;
+		c[0] = leaf_best ? as_leaf : as_node;
+		for (int i = 2; i < kCostRow; ++i) c[i - 1] = fmin(c[i - 2], best_split(cl, cr, i, kb));
+		c[kCostRow - 1] = leaf_best ? 1.0 : 0.0;
+		st_row(a.cost, p, c);
+		p = t.parent[p];
+	}
+}
+
 struct CollapseArgs {
 	BinTree t;
 	uint32_t *misc;
@@ -660,6 +778,7 @@ struct CollapseArgs {
 	NodeF *nf;
 	const uint64_t *keys;   // sorted keys: the scene of binary node b is keys[first[b]] >> 30
 	const uint32_t *param;  // per scene leaf_size | width << 8, or null: leaf_size and width below
+	const double *cost;     // collapse rows (cost-optimal collapse), or null: greedy collapse
 	int leaf_size;
 	int width;
 };
@@ -694,18 +813,52 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 			ne = 1;
 		}
 		int leaf_size = a.leaf_size, width = a.width;
-		if (a.param && b != kNone) {
-			const uint32_t pr = a.param[a.keys[t.first[b]] >> kSceneShift];
-			leaf_size = static_cast<int>(pr & 0xFFu);
-			width = static_cast<int>((pr >> 8) & 0xFFu);
+		if (b != kNone) node_shape(t, a.keys, a.param, b, a.leaf_size, a.width, leaf_size, width);
+		else {
+			leaf_size = max(1, min(kMaxLeaf, leaf_size));
+			width = max(2, min(8, width));
 		}
-		leaf_size = max(1, min(kMaxLeaf, leaf_size));
-		width = max(2, min(8, width));
+		// is_leaf bit k: child k becomes a leaf (by the cost rows, or by its size when greedy)
+		uint32_t is_leaf = 0;
+		if (a.cost && ne == 2) {
+			// the width slots split between the two subtrees where that costs least, and each
+			// part again wherever using fewer slots does not cost as little (at most 8 parts)
+			const double *cost = a.cost;
+			auto row = [cost](uint32_t x) { return cost + static_cast<uint64_t>(kCostRow) * x; };
+			uint32_t sn[8];
+			int sj[8], sp = 0, k;
+			best_split(row(el[0]), row(el[1]), width, k);
+			sn[sp] = el[1];
+			sj[sp++] = width - k;
+			sn[sp] = el[0];
+			sj[sp++] = k;
+			ne = 0;
+			while (sp > 0) {
+				--sp;
+				const uint32_t x = sn[sp];
+				int slots = sj[sp];
+				const double *cx = row(x);
+				while (x < n_int && slots > 1 && cx[slots - 1] == cx[slots - 2]) --slots;
+				if (x >= n_int || slots == 1) {
+					if (x >= n_int || cx[kCostRow - 1] != 0.0) is_leaf |= 1u << ne;
+					el[ne++] = x;
+					continue;
+				}
+				const uint32_t xl = t.left[x], xr = t.right[x];
+				best_split(row(xl), row(xr), slots, k);
+				sn[sp] = xr;
+				sj[sp++] = slots - k;
+				sn[sp] = xl;
+				sj[sp++] = k;
+			}
+		} else if (ne == 1) {
+			is_leaf = 1u;  // a one-triangle scene
+		}
 #pragma unroll
 		for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
 		for (int k = 0; k < ne; ++k)
 			if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
-		while (ne < width) {
+		while (!a.cost && ne < width) {
 			int best = -1;
 			float ba = -1.0f;
 #pragma unroll
@@ -723,13 +876,14 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 			ar[ne] = (r < n_int) ? half_area(t.box[2ull * r], t.box[2ull * r + 1]) : -1.0f;
 			++ne;
 		}
-		// classify: subtrees of at most leaf_size triangles become leaves
+		// classify: greedy collapse makes subtrees of at most leaf_size triangles leaves
 		uint32_t cnt[8];
 		int n_inner = 0;
 		uint32_t n_leaf_tris = 0;
 		for (int k = 0; k < ne; ++k) {
 			cnt[k] = t.count[el[k]];
-			if (cnt[k] > static_cast<uint32_t>(leaf_size)) ++n_inner;
+			if (!a.cost && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
+			if (!((is_leaf >> k) & 1u)) ++n_inner;
 			else n_leaf_tris += cnt[k];
 		}
 		const uint32_t child_base = n_inner ? atomicAdd(a.misc + M_NODES, static_cast<uint32_t>(n_inner)) : 0u;
@@ -771,7 +925,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 				const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
 				lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
 				hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
-				if (cnt[k] > static_cast<uint32_t>(leaf_size)) {
+				if (!((is_leaf >> k) & 1u)) {
 					meta = static_cast<uint8_t>(kMetaInternal | inner);
 					imask |= static_cast<uint8_t>(1u << k);
 					a.bin_of[child_base + inner] = el[k];
@@ -1023,6 +1177,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	PRT_TRY(s.bbox.ensure(sizeof(float4) * 2ull * nb));
 	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * cap));
 	PRT_TRY(s.tri_dest.ensure(sizeof(uint32_t) * n));
+	if (p.collapse == 1) PRT_TRY(s.cost.ensure(sizeof(double) * kCostRow * nb));
 	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * words));
 	PRT_TRY(s.scene_root.ensure(sizeof(uint32_t) * S));
 	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * misc_words(1)));
@@ -1105,6 +1260,18 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
 		                   s.vals[cur].as<uint32_t>(), t);
 	}
+	const int leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size)), width = std::max(2, std::min(8, p.width));
+	if (p.collapse == 1) {
+		if (n > 1) hipLaunchKernelGGL(k_fill, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, t.flag, n - 1, 0u);
+		CostArgs ka;
+		ka.t = t;
+		ka.cost = s.cost.as<double>();
+		ka.keys = keys;
+		ka.param = ss.param;
+		ka.leaf_size = leaf_size;
+		ka.width = width;
+		hipLaunchKernelGGL(k_sah_cost, dim3(blocks_for(n)), dim3(kBlock), 0, st, ka);
+	}
 	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
 	hipLaunchKernelGGL(k_roots, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, s.bin_of.as<uint32_t>(), scene_root, S);
 	CollapseArgs ca;
@@ -1116,8 +1283,9 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	ca.nf = out.nodes_f.as<NodeF>();
 	ca.keys = keys;
 	ca.param = ss.param;
-	ca.leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size));
-	ca.width = std::max(2, std::min(8, p.width));
+	ca.cost = p.collapse == 1 ? s.cost.as<double>() : nullptr;
+	ca.leaf_size = leaf_size;
+	ca.width = width;
 	const int max_levels = fixed ? ss.levels : 128;
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
 	for (int level = 0; level < max_levels; ++level) {

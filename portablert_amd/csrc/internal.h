@@ -67,6 +67,7 @@ struct BuildScratch {
 	DevBuf scene_root;  // binary root node of every scene
 	DevBuf ploc;        // clustering builder: cluster lists, neighbours, flags
 	DevBuf refit_box;   // refit: float4[2 * n_nodes] node boxes
+	DevBuf cost;        // cost-optimal collapse: 8 doubles per binary node
 	void *host_misc = nullptr;  // pinned mirror of the counters and scene 0's bounds
 };
 
@@ -75,6 +76,7 @@ struct BuildParams {
 	int width = 8;      // children per BVH8 node, 2..8 (the node format has 8 slots)
 	int builder = 1;    // binary tree: 0 radix tree over the Morton order (Karras), 1 clustering
 	                    // (PLOC) over the same order; graph-capturable builds use the radix tree
+	int collapse = 1;   // binary tree to BVH8: 0 greedy largest-area opening, 1 least SAH cost
 };
 
 constexpr uint32_t kMaxScenes = 65536;

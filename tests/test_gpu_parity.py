@@ -394,3 +394,27 @@ def test_bvh_invariants(gpu_ctx):
     gpu_ctx.set_tris(tris)
     check_bvh(gpu_ctx, tris)
 
+
+
+@pytest.mark.parametrize("builder,collapse", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_builders_and_collapse(gpu_ctx, builder, collapse):
+    """Radix-tree and clustering builders, greedy and least-cost collapse: a valid BVH of the room
+    and the same records as the oracle and the reference on its golden sample."""
+    gpu_ctx.set_option("builder", builder)
+    gpu_ctx.set_option("collapse", collapse)
+    assert gpu_ctx.get_option("collapse") == collapse
+    with pytest.raises(prt.PrtError) as e:
+        gpu_ctx.set_option("collapse", 2)
+    assert e.value.code == prt.ERR_ARG
+    try:
+        tris = scenes.scene(2)
+        g = load_golden("workload2")
+        print(f"builder {builder} collapse {collapse}", flush=True)
+        gpu_ctx.set_tris(tris)
+        check_bvh(gpu_ctx, tris)
+        got = gpu_ctx.nearest_hits(g["rays"])
+        assert_same(got, oracle.trace(tris, g["rays"], oracle.LOWEST_ID), what="vs oracle")
+        assert_same(got, as_full(g["hits"]), ids=False, what="vs reference")
+    finally:
+        gpu_ctx.set_option("builder", 1)
+        gpu_ctx.set_option("collapse", 1)
