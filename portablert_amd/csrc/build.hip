@@ -434,8 +434,136 @@ __device__ __forceinline__ void ld_box(const float4 *box, uint32_t node, float4 
 // visible to every CU), so the flag increment that follows publishes them.
 __device__ __forceinline__ void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// ------------------------------------------- collapse costs (surface-area heuristic, SAH)
+//
+// Cost-optimal collapse of the binary tree into nodes of at most `width` children, by the
+// dynamic program of Ylitie, Karras and Laine 2017 ("Efficient incoherent ray traversal on GPUs
+// through compressed wide BVHs", section 3.1). For binary node x and i in 1..7 slots, C(x, i) is
+// the least expected cost of x's subtree when it may fill i child slots of the node above it:
+//   C(x, 1) = min(leaf(x), A(x) * kCostNode + split(x, width))
+//   C(x, i) = min(C(x, i - 1), split(x, i))                               for i >= 2
+//   split(x, i) = min over k in 1..i-1 of C(left(x), k) + C(right(x), i - k)
+//   leaf(x) = A(x) * kCostTri * P(x) if x holds at most leaf_size triangles, else +inf
+// with A the half surface area of x's box relative to its scene's box (computed in double, so
+// coordinates near 1e19 or 1e-19 neither overflow nor vanish) and P the triangle count; a single
+// triangle costs A * kCostTri in any number of slots. A row holds C(x, 1..7) and, last, 1 when x
+// is best kept as one leaf. Rows of internal nodes are computed where the tree is built (at the
+// merge for the clustered tree, in the bottom-up box pass for the radix tree); rows of binary
+// leaves are recomputed from their boxes wherever they are needed.
+constexpr float kCostNode = 1.0f;
+constexpr float kCostTri = 0.3f;
+constexpr int kCostRow = 8;
+
+struct CostCtx {
+	float *row;             // kCostRow floats per binary internal node, or null: greedy collapse
+	const double *scale;    // per scene: 1 / half-area of the scene box
+	const uint64_t *keys;   // sorted keys: the scene of binary node b is keys[first[b]] >> 30
+	const uint32_t *param;  // per scene leaf_size | width << 8, or null: leaf_size and width below
+	int leaf_size;
+	int width;
+};
+
+// Leaf size and node width of scene `scene` (per-scene parameters when given).
+__device__ __forceinline__ void scene_shape(const CostCtx &c, uint32_t scene, int &ls, int &w) {
+	int leaf_size = c.leaf_size, width = c.width;
+	if (c.param) {
+		const uint32_t pr = c.param[scene];
+		leaf_size = static_cast<int>(pr & 0xFFu);
+		width = static_cast<int>((pr >> 8) & 0xFFu);
+	}
+	ls = max(1, min(kMaxLeaf, leaf_size));
+	w = max(2, min(8, width));
+}
+
+// Scene of binary node x (leaf j holds sorted key j; an internal node, its first key's).
+__device__ __forceinline__ uint32_t scene_of(const BinTree &t, const CostCtx &c, uint32_t x) {
+	const uint32_t k = x >= t.n - 1 ? x - (t.n - 1) : t.first[x];
+	return static_cast<uint32_t>(c.keys[k] >> kSceneShift);
+}
+
+// Half-area of [lo, hi] relative to the scene's.
+__device__ __forceinline__ float rel_area(float4 lo, float4 hi, double scale) {
+	const double dx = static_cast<double>(hi.x) - lo.x, dy = static_cast<double>(hi.y) - lo.y,
+	             dz = static_cast<double>(hi.z) - lo.z;
+	return static_cast<float>((dx * dy + dx * dz + dy * dz) * scale);
+}
+
+__device__ __forceinline__ void leaf_row(float area, float r[kCostRow]) {
+#pragma unroll
+	for (int k = 0; k < kCostRow - 1; ++k) r[k] = area * kCostTri;
+	r[kCostRow - 1] = 1.0f;
+}
+
+// split(x, i) from the rows of x's children, and the k that attains it (the smallest on ties).
+__device__ __forceinline__ float best_split(const float *cl, const float *cr, int i, int &kbest) {
+	float best = cl[0] + cr[i - 2];
+	kbest = 1;
+	for (int k = 2; k < i; ++k) {
+		const float v = cl[k - 1] + cr[i - k - 1];
+		if (v < best) {
+			best = v;
+			kbest = k;
+		}
+	}
+	return best;
+}
+
+// Row of an internal node from its area, triangle count, shape and its children's rows.
+__device__ __forceinline__ void node_row(float area, uint32_t count, int ls, int w, const float cl[kCostRow],
+                                         const float cr[kCostRow], float c[kCostRow]) {
+	int kb;
+	const bool can_leaf = count <= static_cast<uint32_t>(ls);
+	const float as_leaf = can_leaf ? area * kCostTri * static_cast<float>(count) : __builtin_inff();
+	const float as_node = area * kCostNode + best_split(cl, cr, w, kb);
+	const bool leaf_best = can_leaf && as_leaf <= as_node;
+	c[0] = leaf_best ? as_leaf : as_node;
+	for (int i = 2; i < kCostRow; ++i) c[i - 1] = fminf(c[i - 2], best_split(cl, cr, i, kb));
+	c[kCostRow - 1] = leaf_best ? 1.0f : 0.0f;
+}
+
+// Rows written by one thread and read by a thread of another workgroup in the same launch go
+// through device-scope atomic 8-byte stores and loads, as the boxes do.
+__device__ __forceinline__ void st_row(float *rows, uint32_t x, const float c[kCostRow]) {
+	uint64_t *p = reinterpret_cast<uint64_t *>(rows + static_cast<uint64_t>(kCostRow) * x);
+#pragma unroll
+	for (int k = 0; k < kCostRow / 2; ++k)
+		__hip_atomic_store(p + k, pack2(c[2 * k], c[2 * k + 1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ld_row(const float *rows, uint32_t x, float c[kCostRow]) {
+	const uint64_t *p = reinterpret_cast<const uint64_t *>(rows + static_cast<uint64_t>(kCostRow) * x);
+#pragma unroll
+	for (int k = 0; k < kCostRow / 2; ++k) {
+		const uint64_t v = __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		c[2 * k] = __uint_as_float(static_cast<uint32_t>(v));
+		c[2 * k + 1] = __uint_as_float(static_cast<uint32_t>(v >> 32));
+	}
+}
+
+// Row of binary node x written in an earlier launch (plain loads), or computed for a leaf.
+__device__ __forceinline__ void row_of(const BinTree &t, const CostCtx &c, uint32_t x, float r[kCostRow]) {
+	if (x >= t.n - 1) {
+		leaf_row(rel_area(t.box[2ull * x], t.box[2ull * x + 1], c.scale[scene_of(t, c, x)]), r);
+		return;
+	}
+	const float4 *p = reinterpret_cast<const float4 *>(c.row + static_cast<uint64_t>(kCostRow) * x);
+	const float4 a = p[0], b = p[1];
+	r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+	r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+}
+
+// 1 / half-area of every scene's box (1 for an empty or flat scene).
+__global__ __launch_bounds__(kBlock) void k_scene_scale(const uint32_t *misc, uint32_t scenes, double *scale) {
+	const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s >= scenes) return;
+	const uint32_t *b = misc + M_SCENE + M_PER_SCENE * s + 6;
+	const double dx = static_cast<double>(o2f(b[3])) - o2f(b[0]), dy = static_cast<double>(o2f(b[4])) - o2f(b[1]),
+	             dz = static_cast<double>(o2f(b[5])) - o2f(b[2]);
+	const double a = dx * dy + dx * dz + dy * dz;
+	scale[s] = (a > 0.0 && a < 1e300) ? 1.0 / a : 1.0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ tbox,
-                                                     const uint32_t *__restrict__ sorted, BinTree t) {
+                                                     const uint32_t *__restrict__ sorted, BinTree t, CostCtx cc) {
 	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
 	if (j >= t.n) return;
 	uint32_t node = t.n - 1 + j;
@@ -445,6 +573,8 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	st_box(t.box, node, lo, hi);
 	t.count[node] = 1;
 	t.first[node] = j;
+	float c[kCostRow];  // this node's collapse row (when costs are on)
+	if (cc.row) leaf_row(rel_area(lo, hi, cc.scale[scene_of(t, cc, node)]), c);
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
 	// a radix tree over at most 2^29 keys of at most 46 bits is at most 46 + 29 levels deep
 	for (int guard = 0; p != kNone && guard < 128; ++guard) {
@@ -460,6 +590,19 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 		hi.y = fmaxf(hi.y, sh.y);
 		hi.z = fmaxf(hi.z, sh.z);
 		st_box(t.box, p, lo, hi);
+		if (cc.row) {
+			float cs[kCostRow], cp[kCostRow];
+			if (sib >= t.n - 1) leaf_row(rel_area(sl, sh, cc.scale[scene_of(t, cc, sib)]), cs);
+			else ld_row(cc.row, sib, cs);
+			const uint32_t scene = scene_of(t, cc, p);
+			int ls, w;
+			scene_shape(cc, scene, ls, w);
+			const bool left = t.left[p] == node;
+			node_row(rel_area(lo, hi, cc.scale[scene]), t.count[p], ls, w, left ? c : cs, left ? cs : c, cp);
+#pragma unroll
+			for (int k = 0; k < kCostRow; ++k) c[k] = cp[k];
+			st_row(cc.row, p, c);
+		}
 		node = p;
 		p = t.parent[p];
 	}
@@ -494,11 +637,6 @@ __global__ __launch_bounds__(kBlock) void k_ploc_init(const float4 *__restrict__
 	t.box[2ull * node + 1] = tbox[2ull * tri + 1];
 	t.count[node] = 1u;
 	t.first[node] = j;
-	t.parent[node] = kNone;  // scene roots keep kNone; merges set the others
-	if (j + 1 < t.n) {
-		t.parent[j] = kNone;
-		t.flag[j] = 0u;
-	}
 	clu[j] = node;
 }
 
@@ -550,7 +688,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *__restrict__
 // Mutual pairs (i < j) merge at position i; the partner's position is dropped by the compaction.
 __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restrict__ clu, const int32_t *__restrict__ nn,
                                                        BinTree t, uint32_t *__restrict__ clu_next,
-                                                       uint32_t *__restrict__ flags, uint32_t n_flags, uint32_t *misc) {
+                                                       uint32_t *__restrict__ flags, uint32_t n_flags, uint32_t *misc,
+                                                       CostCtx cc) {
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n_flags || misc[M_PDONE]) return;
 	const uint32_t m = misc[M_PM];
@@ -571,11 +710,24 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restric
 				t.box[2ull * id + 1] = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z), 0.0f);
 				t.left[id] = l;
 				t.right[id] = r;
-				t.parent[l] = id;
-				t.parent[r] = id;
 				t.count[id] = t.count[l] + t.count[r];
 				t.first[id] = t.first[l];
 				node = id;
+				if (cc.row) {
+					// the children are leaves or merged in earlier steps: their rows are final
+					float cl[kCostRow], cr[kCostRow], cp[kCostRow];
+					row_of(t, cc, l, cl);
+					row_of(t, cc, r, cr);
+					const uint32_t scene = scene_of(t, cc, l);
+					int ls, w;
+					scene_shape(cc, scene, ls, w);
+					const float4 plo = make_float4(fminf(la.x, ra.x), fminf(la.y, ra.y), fminf(la.z, ra.z), 0.0f);
+					const float4 phi = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z), 0.0f);
+					node_row(rel_area(plo, phi, cc.scale[scene]), t.count[l] + t.count[r], ls, w, cl, cr, cp);
+					float4 *dst = reinterpret_cast<float4 *>(cc.row + static_cast<uint64_t>(kCostRow) * id);
+					dst[0] = make_float4(cp[0], cp[1], cp[2], cp[3]);
+					dst[1] = make_float4(cp[4], cp[5], cp[6], cp[7]);
+				}
 			} else {
 				atomicOr(misc + M_ERR, 2u);  // more merges than internal nodes: never happens
 			}
@@ -658,117 +810,6 @@ __device__ __forceinline__ int grid_exp(float p, float top) {
 	return e;
 }
 
-// Leaf size and node width of the scene of binary node b (per-scene parameters when given).
-__device__ __forceinline__ void node_shape(const BinTree &t, const uint64_t *keys, const uint32_t *param, uint32_t b,
-                                           int leaf_size, int width, int &ls, int &w) {
-	if (param) {
-		const uint32_t pr = param[keys[t.first[b]] >> kSceneShift];
-		leaf_size = static_cast<int>(pr & 0xFFu);
-		width = static_cast<int>((pr >> 8) & 0xFFu);
-	}
-	ls = max(1, min(kMaxLeaf, leaf_size));
-	w = max(2, min(8, width));
-}
-
-// ------------------------------------------- collapse costs (surface-area heuristic, SAH)
-//
-// Cost-optimal collapse of the binary tree into nodes of at most `width` children, by the
-// dynamic program of Ylitie, Karras and Laine 2017 ("Efficient incoherent ray traversal on GPUs
-// through compressed wide BVHs", section 3.1). For binary node x and i in 1..7 slots, C(x, i) is
-// the least expected cost of x's subtree when it may fill i child slots of the node above it:
-//   C(x, 1) = min(leaf(x), A(x) * kCostNode + split(x, width))
-//   C(x, i) = min(C(x, i - 1), split(x, i))                               for i >= 2
-//   split(x, i) = min over k in 1..i-1 of C(left(x), k) + C(right(x), i - k)
-//   leaf(x) = A(x) * kCostTri * P(x) if x holds at most leaf_size triangles, else +inf
-// with A the half surface area of x's box and P its triangle count; a single triangle costs
-// A * kCostTri in any number of slots. The greedy alternative (open the largest child until the
-// node is full) is kept as collapse mode 0. Costs are doubles: fp32 areas overflow for
-// coordinates near 1e19 and vanish near 1e-19.
-constexpr double kCostNode = 1.0;
-constexpr double kCostTri = 0.3;
-constexpr int kCostRow = 8;  // C(x, 1..7), then 1.0 if x is best kept as one leaf, else 0.0
-
-__device__ __forceinline__ double dhalf_area(float4 lo, float4 hi) {
-	const double dx = static_cast<double>(hi.x) - lo.x, dy = static_cast<double>(hi.y) - lo.y,
-	             dz = static_cast<double>(hi.z) - lo.z;
-	return dx * dy + dx * dz + dy * dz;
-}
-
-// split(x, i) from the rows of x's children, and the k that attains it (the smallest on ties).
-__device__ __forceinline__ double best_split(const double *cl, const double *cr, int i, int &kbest) {
-	double best = cl[0] + cr[i - 2];
-	kbest = 1;
-	for (int k = 2; k < i; ++k) {
-		const double v = cl[k - 1] + cr[i - k - 1];
-		if (v < best) {
-			best = v;
-			kbest = k;
-		}
-	}
-	return best;
-}
-
-// Rows written by one thread and read by a thread of another workgroup in the same launch:
-// device-scope atomic 8-byte stores and loads, as for the boxes.
-__device__ __forceinline__ void st_row(double *cost, uint32_t x, const double c[kCostRow]) {
-	uint64_t *p = reinterpret_cast<uint64_t *>(cost + static_cast<uint64_t>(kCostRow) * x);
-#pragma unroll
-	for (int k = 0; k < kCostRow; ++k)
-		__hip_atomic_store(p + k, static_cast<uint64_t>(__double_as_longlong(c[k])), __ATOMIC_RELAXED,
-		                   __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void ld_row(const double *cost, uint32_t x, double c[kCostRow]) {
-	const uint64_t *p = reinterpret_cast<const uint64_t *>(cost + static_cast<uint64_t>(kCostRow) * x);
-#pragma unroll
-	for (int k = 0; k < kCostRow; ++k)
-		c[k] = __longlong_as_double(static_cast<long long>(__hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-}
-
-struct CostArgs {
-	BinTree t;
-	double *cost;  // kCostRow doubles per binary node
-	const uint64_t *keys;
-	const uint32_t *param;
-	int leaf_size;
-	int width;
-};
-
-// Rows leaf to root: every triangle starts at its leaf, the second child to finish computes the
-// parent (flags zero on entry, parents kNone at the roots).
-__global__ __launch_bounds__(kBlock) void k_sah_cost(CostArgs a) {
-	const BinTree &t = a.t;
-	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-	if (j >= t.n) return;
-	const uint32_t x = t.n - 1 + j;
-	double c[kCostRow];
-	const double leaf = dhalf_area(t.box[2ull * x], t.box[2ull * x + 1]) * kCostTri;
-#pragma unroll
-	for (int k = 0; k < kCostRow - 1; ++k) c[k] = leaf;
-	c[kCostRow - 1] = 1.0;
-	st_row(a.cost, x, c);
-	uint32_t p = t.n > 1 ? t.parent[x] : kNone;
-	for (uint32_t guard = 0; p != kNone && guard < t.n; ++guard) {
-		stores_done();
-		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
-		double cl[kCostRow], cr[kCostRow];
-		ld_row(a.cost, t.left[p], cl);
-		ld_row(a.cost, t.right[p], cr);
-		int ls, w, kb;
-		node_shape(t, a.keys, a.param, p, a.leaf_size, a.width, ls, w);
-		const double area = dhalf_area(t.box[2ull * p], t.box[2ull * p + 1]);
-		const uint32_t count = t.count[p];
-		const bool can_leaf = count <= static_cast<uint32_t>(ls);
-		const double as_leaf = can_leaf ? area * kCostTri * count : __builtin_inf();
-		const double as_node = area * kCostNode + best_split(cl, cr, w, kb);
-		const bool leaf_best = can_leaf && as_leaf <= as_node;
-		c[0] = leaf_best ? as_leaf : as_node;
-		for (int i = 2; i < kCostRow; ++i) c[i - 1] = fmin(c[i - 2], best_split(cl, cr, i, kb));
-		c[kCostRow - 1] = leaf_best ? 1.0 : 0.0;
-		st_row(a.cost, p, c);
-		p = t.parent[p];
-	}
-}
-
 struct CollapseArgs {
 	BinTree t;
 	uint32_t *misc;
@@ -776,11 +817,7 @@ struct CollapseArgs {
 	uint32_t *tri_dest;
 	NodeQ *nq;
 	NodeF *nf;
-	const uint64_t *keys;   // sorted keys: the scene of binary node b is keys[first[b]] >> 30
-	const uint32_t *param;  // per scene leaf_size | width << 8, or null: leaf_size and width below
-	const double *cost;     // collapse rows (cost-optimal collapse), or null: greedy collapse
-	int leaf_size;
-	int width;
+	CostCtx cc;  // keys, per-scene shape; cc.row null: greedy collapse
 };
 
 __global__ __launch_bounds__(kBlock) void k_level_advance(uint32_t *misc) {
@@ -812,22 +849,19 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 			el[0] = b;
 			ne = 1;
 		}
-		int leaf_size = a.leaf_size, width = a.width;
-		if (b != kNone) node_shape(t, a.keys, a.param, b, a.leaf_size, a.width, leaf_size, width);
-		else {
-			leaf_size = max(1, min(kMaxLeaf, leaf_size));
-			width = max(2, min(8, width));
-		}
+		int leaf_size, width;
+		scene_shape(a.cc, b != kNone ? scene_of(t, a.cc, b) : 0u, leaf_size, width);
 		// is_leaf bit k: child k becomes a leaf (by the cost rows, or by its size when greedy)
 		uint32_t is_leaf = 0;
-		if (a.cost && ne == 2) {
+		if (a.cc.row && ne == 2) {
 			// the width slots split between the two subtrees where that costs least, and each
 			// part again wherever using fewer slots does not cost as little (at most 8 parts)
-			const double *cost = a.cost;
-			auto row = [cost](uint32_t x) { return cost + static_cast<uint64_t>(kCostRow) * x; };
+			float rl[kCostRow], rr[kCostRow], cx[kCostRow];
 			uint32_t sn[8];
 			int sj[8], sp = 0, k;
-			best_split(row(el[0]), row(el[1]), width, k);
+			row_of(t, a.cc, el[0], rl);
+			row_of(t, a.cc, el[1], rr);
+			best_split(rl, rr, width, k);
 			sn[sp] = el[1];
 			sj[sp++] = width - k;
 			sn[sp] = el[0];
@@ -837,15 +871,22 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 				--sp;
 				const uint32_t x = sn[sp];
 				int slots = sj[sp];
-				const double *cx = row(x);
-				while (x < n_int && slots > 1 && cx[slots - 1] == cx[slots - 2]) --slots;
-				if (x >= n_int || slots == 1) {
-					if (x >= n_int || cx[kCostRow - 1] != 0.0) is_leaf |= 1u << ne;
+				if (x >= n_int) {  // one triangle: a leaf
+					is_leaf |= 1u << ne;
+					el[ne++] = x;
+					continue;
+				}
+				row_of(t, a.cc, x, cx);
+				while (slots > 1 && cx[slots - 1] == cx[slots - 2]) --slots;
+				if (slots == 1) {
+					if (cx[kCostRow - 1] != 0.0f) is_leaf |= 1u << ne;
 					el[ne++] = x;
 					continue;
 				}
 				const uint32_t xl = t.left[x], xr = t.right[x];
-				best_split(row(xl), row(xr), slots, k);
+				row_of(t, a.cc, xl, rl);
+				row_of(t, a.cc, xr, rr);
+				best_split(rl, rr, slots, k);
 				sn[sp] = xr;
 				sj[sp++] = slots - k;
 				sn[sp] = xl;
@@ -858,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 		for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
 		for (int k = 0; k < ne; ++k)
 			if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
-		while (!a.cost && ne < width) {
+		while (!a.cc.row && ne < width) {
 			int best = -1;
 			float ba = -1.0f;
 #pragma unroll
@@ -882,7 +923,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 		uint32_t n_leaf_tris = 0;
 		for (int k = 0; k < ne; ++k) {
 			cnt[k] = t.count[el[k]];
-			if (!a.cost && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
+			if (!a.cc.row && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
 			if (!((is_leaf >> k) & 1u)) ++n_inner;
 			else n_leaf_tris += cnt[k];
 		}
@@ -1177,7 +1218,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	PRT_TRY(s.bbox.ensure(sizeof(float4) * 2ull * nb));
 	PRT_TRY(s.bin_of.ensure(sizeof(uint32_t) * cap));
 	PRT_TRY(s.tri_dest.ensure(sizeof(uint32_t) * n));
-	if (p.collapse == 1) PRT_TRY(s.cost.ensure(sizeof(double) * kCostRow * nb));
+	const uint64_t scale_words = (S + 1ull) & ~1ull;  // doubles, keeps the rows 16-byte aligned
+	if (p.collapse == 1) PRT_TRY(s.cost.ensure(sizeof(double) * scale_words + sizeof(float) * kCostRow * n));
 	PRT_TRY(s.misc.ensure(sizeof(uint32_t) * words));
 	PRT_TRY(s.scene_root.ensure(sizeof(uint32_t) * S));
 	if (!s.host_misc) PRT_TRY(hipHostMalloc(&s.host_misc, sizeof(uint32_t) * misc_words(1)));
@@ -1225,6 +1267,20 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	t.n = n;
 	uint32_t *scene_root = s.scene_root.as<uint32_t>();
 	uint32_t *hm = static_cast<uint32_t *>(s.host_misc);
+	const int leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size)), width = std::max(2, std::min(8, p.width));
+	CostCtx cc;
+	cc.row = nullptr;
+	cc.scale = nullptr;
+	cc.keys = keys;
+	cc.param = ss.param;
+	cc.leaf_size = leaf_size;
+	cc.width = width;
+	if (p.collapse == 1) {
+		double *scale = s.cost.as<double>();
+		cc.scale = scale;
+		cc.row = reinterpret_cast<float *>(scale + scale_words);
+		hipLaunchKernelGGL(k_scene_scale, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, S, scale);
+	}
 	if (p.builder == 1 && !fixed) {
 		// clustering: steps until one merges nothing, the host checks every 8 steps
 		const uint32_t n_seg_p = blocks_for(n, kScanSeg);
@@ -1240,7 +1296,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 			uint32_t *cin = (it & 1) ? clu1 : clu0, *cout = (it & 1) ? clu0 : clu1;
 			hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(n)), dim3(kBlock), 0, st, cin, t, misc, nnb);
 			hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(n)), dim3(kBlock), 0, st, cin, nnb, t, clu_next, flags, n,
-			                   misc);
+			                   misc, cc);
 			hipLaunchKernelGGL(k_scan_seg, dim3(n_seg_p), dim3(kBlock), 0, st, flags, n, segp);
 			hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, segp, n_seg_p);
 			hipLaunchKernelGGL(k_ploc_compact, dim3(blocks_for(n)), dim3(kBlock), 0, st, clu_next, nnb, flags, segp, cout,
@@ -1258,19 +1314,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		if (n > 1)
 			hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, keys, t, scene_root);
 		hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
-		                   s.vals[cur].as<uint32_t>(), t);
-	}
-	const int leaf_size = std::max(1, std::min(kMaxLeaf, p.leaf_size)), width = std::max(2, std::min(8, p.width));
-	if (p.collapse == 1) {
-		if (n > 1) hipLaunchKernelGGL(k_fill, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, t.flag, n - 1, 0u);
-		CostArgs ka;
-		ka.t = t;
-		ka.cost = s.cost.as<double>();
-		ka.keys = keys;
-		ka.param = ss.param;
-		ka.leaf_size = leaf_size;
-		ka.width = width;
-		hipLaunchKernelGGL(k_sah_cost, dim3(blocks_for(n)), dim3(kBlock), 0, st, ka);
+		                   s.vals[cur].as<uint32_t>(), t, cc);
 	}
 	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
 	hipLaunchKernelGGL(k_roots, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, s.bin_of.as<uint32_t>(), scene_root, S);
@@ -1281,11 +1325,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	ca.tri_dest = s.tri_dest.as<uint32_t>();
 	ca.nq = out.nodes_q.as<NodeQ>();
 	ca.nf = out.nodes_f.as<NodeF>();
-	ca.keys = keys;
-	ca.param = ss.param;
-	ca.cost = p.collapse == 1 ? s.cost.as<double>() : nullptr;
-	ca.leaf_size = leaf_size;
-	ca.width = width;
+	ca.cc = cc;
 	const int max_levels = fixed ? ss.levels : 128;
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
 	for (int level = 0; level < max_levels; ++level) {

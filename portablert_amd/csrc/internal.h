@@ -67,7 +67,7 @@ struct BuildScratch {
 	DevBuf scene_root;  // binary root node of every scene
 	DevBuf ploc;        // clustering builder: cluster lists, neighbours, flags
 	DevBuf refit_box;   // refit: float4[2 * n_nodes] node boxes
-	DevBuf cost;        // cost-optimal collapse: 8 doubles per binary node
+	DevBuf cost;        // cost-optimal collapse: per-scene area scales, 8 floats per internal node
 	void *host_misc = nullptr;  // pinned mirror of the counters and scene 0's bounds
 };
 
