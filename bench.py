@@ -283,7 +283,8 @@ def batch_leg(ctx, steps, warmup, stream):
     """64 scenes of 1,011..191,723 triangles with 10,948..1,979,349 rays each (2,516,661
     triangles, 24,428,377 rays): every BVH build and the one persistent traversal launch replayed
     as one hipGraph, after the autotuner. Compared with the same work scene by scene through
-    set_tris + nearest_hits with the same (radix-tree) builder, a host synchronisation per build."""
+    set_tris + nearest_hits with the same builder and collapse (radix tree, greedy collapse), a host
+    synchronisation per build."""
     import torch
 
     import portablert_amd as prt
@@ -320,8 +321,9 @@ def batch_leg(ctx, steps, warmup, stream):
     tri_off = np.concatenate([[0], np.cumsum(tc)])
     ray_off = np.concatenate([[0], np.cumsum(rc)])
 
-    builder0 = ctx.get_option("builder")
-    ctx.set_option("builder", 0)  # the batch builds radix trees: compare like with like
+    builder0, collapse0 = ctx.get_option("builder"), ctx.get_option("collapse")
+    ctx.set_option("builder", 0)  # the batch builds radix trees with the greedy collapse:
+    ctx.set_option("collapse", 0)  # compare like with like
 
     def sequential():
         for s in range(n_sc):
@@ -336,6 +338,7 @@ def batch_leg(ctx, steps, warmup, stream):
         sequential()
     seq_ms = (time.perf_counter() - t0) * 1e3 / reps
     ctx.set_option("builder", builder0)
+    ctx.set_option("collapse", collapse0)
     identical = bool(torch.equal(d_out, d_seq))
     log(f"batch: graph {graph_ms:.2f} ms, eager {eager_ms:.2f} ms, scene by scene {seq_ms:.2f} ms; "
         f"tuner {before:.2f} -> {after:.2f} ms in {tune_s:.1f} s; identical {identical}")
