@@ -244,8 +244,10 @@ int prt_mi355x_set_tris_device(prt_ctx *c, const float *d_tris, uint64_t n) {
 static int trace_common(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t mask, void *d_out, hipStream_t st) {
 	float ms = 0.0f;
 	prt::TraceStats stats;
-	hipError_t e = prt::trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st,
-	                               c->time_kernel ? &ms : nullptr, c->tp.count_stats ? &stats : nullptr);
+	hipError_t e = prt::order_use(c, st);
+	if (e == hipSuccess)
+		e = prt::trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st, c->time_kernel ? &ms : nullptr,
+		                    c->tp.count_stats ? &stats : nullptr);
 	if (e == hipSuccess) e = prt::mark_use(c, st);
 	if (e != hipSuccess) return hip_fail(c, e, "nearest_hits: traversal");
 	if (c->time_kernel) c->trace_ms = ms;

@@ -16,9 +16,11 @@
 struct prt_ctx {
 	int device = 0;
 	hipStream_t stream = nullptr;
-	// recorded after a traversal enqueued on a caller stream; rebuilding the BVH waits for it
+	// recorded after every traversal, on the stream it was enqueued on (busy_stream): rebuilding
+	// the BVH and traversals on other streams wait for it
 	hipEvent_t busy = nullptr;
 	bool busy_set = false;
+	hipStream_t busy_stream = nullptr;
 	std::string name;
 	std::string err;
 	prt::Bvh8 bvh;
@@ -60,18 +62,27 @@ inline int fail(prt_ctx *c, int code, const std::string &msg) {
 	return code;
 }
 
-// Marks the BVH as read by work enqueued on `st` (when st is not the context's stream).
+// Marks the BVH, the traversal scratch and the learned buffers as used by work enqueued on `st`.
 inline hipError_t mark_use(prt_ctx *c, hipStream_t st) {
-	if (st == c->stream || !c->busy) return hipSuccess;
+	if (!c->busy) return hipSuccess;
 	c->busy_set = true;
+	c->busy_stream = st;
 	return hipEventRecord(c->busy, st);
 }
 
-// Makes the context's stream wait for the last traversal enqueued on a caller stream.
+// Makes `st` wait for the last traversal when it was enqueued on another stream: the launches
+// of one context share its work counter, stack spill buffer and learned buffers, so calls on
+// different streams run one after the other.
+inline hipError_t order_use(prt_ctx *c, hipStream_t st) {
+	if (!c->busy_set || c->busy_stream == st) return hipSuccess;
+	return hipStreamWaitEvent(st, c->busy, 0);
+}
+
+// Makes the context's stream wait for the last traversal (before the BVH is rebuilt or moved).
 inline hipError_t wait_use(prt_ctx *c) {
-	if (!c->busy_set) return hipSuccess;
+	const hipError_t e = order_use(c, c->stream);
 	c->busy_set = false;
-	return hipStreamWaitEvent(c->stream, c->busy, 0);
+	return e;
 }
 
 // Runs an entry point's body; a failed host allocation (or any other C++ exception) becomes an

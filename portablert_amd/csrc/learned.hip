@@ -177,7 +177,9 @@ hipError_t build_clusters(prt_ctx *c) {
 // Labels for n device rays from the plain traversal.
 hipError_t make_labels(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t *labels) {
 	PRT_TRY(c->hits_tmp.ensure(16 * n));
+	PRT_TRY(order_use(c, c->stream));
 	PRT_TRY(trace_bvh8(c->bvh, c->ts, c->tp, d_rays, n, F_T | F_ID, c->hits_tmp.ptr, c->stream, nullptr, nullptr));
+	PRT_TRY(mark_use(c, c->stream));
 	hipLaunchKernelGGL(k_labels, dim3(blocks(n)), dim3(256), 0, c->stream, c->hits_tmp.as<uint4>(), n,
 	                   c->clusters.tri_cluster.as<uint32_t>(), static_cast<uint32_t>(c->bvh.n_tris), labels);
 	return hipGetLastError();
@@ -277,8 +279,8 @@ int prt_mi355x_nearest_hits_learned_device(prt_ctx *c, const float *d_rays, uint
 	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
 	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-	hipError_t e = hipSuccess;
-	if (c->time_kernel) {
+	hipError_t e = order_use(c, st);  // the inference buffers and the traversal scratch are shared
+	if (e == hipSuccess && c->time_kernel) {
 		e = hipEventCreate(&e0);
 		if (e == hipSuccess) e = hipEventCreate(&e1);
 		if (e == hipSuccess) e = hipEventCreate(&e2);

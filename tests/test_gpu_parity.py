@@ -207,6 +207,34 @@ def test_device_entry_point_matches_host(gpu_ctx):
     assert dev.tobytes() == host.tobytes()
 
 
+def test_calls_on_two_streams(gpu_ctx):
+    """Device calls enqueued back to back on two streams with no host synchronisation between
+    them: the context orders them (they share its work counter and stack scratch), and a host
+    call afterwards waits for both."""
+    import torch
+    tris = scenes.scene(2)
+    gpu_ctx.set_tris(tris)
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    sets = [scenes.rays(2, first=k * 600000, count=600000 - 77 * k) for k in range(3)]
+    d_rays = [torch.from_numpy(r).cuda() for r in sets]
+    d_out = [torch.zeros(len(r) * size, dtype=torch.uint8, device="cuda") for r in sets]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]  # default priority
+    torch.cuda.synchronize()
+    print("two streams: 3 calls", [len(r) for r in sets], flush=True)
+    for k in range(3):
+        gpu_ctx.nearest_hits_device(d_rays[k].data_ptr(), len(sets[k]), mask, d_out[k].data_ptr(),
+                                    streams[k % 2].cuda_stream)
+    host = gpu_ctx.nearest_hits(sets[0][:1000], mask)
+    torch.cuda.synchronize()
+    for k in range(3):
+        got = d_out[k].cpu().numpy().view(prt.hit_dtype(mask))
+        idx = np.arange(0, len(sets[k]), 97)
+        want = oracle.trace(tris, sets[k][idx], oracle.LOWEST_ID)
+        assert_same(got[idx], want, what=f"call {k} (stream {k % 2})")
+    assert host.tobytes() == d_out[0].cpu().numpy().view(prt.hit_dtype(mask))[:1000].tobytes()
+
+
 def test_host_entry_point_in_chunks(gpu_ctx):
     """Host calls above one chunk (1,048,576 rays) are pipelined through pinned buffers: a ragged
     count with the full record layout equals the device entry point byte for byte."""
