@@ -126,8 +126,11 @@ int prt_mi355x_nearest_hits(prt_ctx *ctx, const float *rays6, uint64_t n, uint32
                             uint32_t stride, const int32_t off[8]);
 
 /* Same with rays and records in device memory; enqueued on `stream` (hipStream_t, NULL = the
- * context's stream) without synchronising unless PRT_OPT_TIME_KERNEL is set. No counterpart in
- * the reference, which copies on every call (src/intersect_hip.cpp:322-339). */
+ * context's stream) without synchronising unless PRT_OPT_TIME_KERNEL is set. Calls of one
+ * context run in order: a call on another stream than the previous call's waits for it on the
+ * device (they share the context's scratch); use one context per stream to run traversals
+ * concurrently. No counterpart in the reference, which copies on every call
+ * (src/intersect_hip.cpp:322-339). */
 int prt_mi355x_nearest_hits_device(prt_ctx *ctx, const float *d_rays6, uint64_t n, uint32_t mask,
                                    void *d_out, uint32_t stride, const int32_t off[8], void *stream);
 
