@@ -378,15 +378,17 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 }
 
 // Waves per SIMD the register allocation must allow. Quantized nodes with an 8-entry LDS stack
-// (4 KB per wave): 8, i.e. at most 64 VGPRs; the traversal is partly latency-bound (dependent
-// node fetches) and the extra waves hide more of it than the few spilled registers cost. A
-// 16-entry stack allows 5 waves by LDS alone. The fp32 node test needs about 100 VGPRs and is
-// left to the compiler (1 = no constraint).
-template <class Node, int S>
-constexpr int kWavesPerEu = std::is_same<Node, NodeQ>::value ? (S == 8 ? 8 : 5) : 1;
+// (4 KB per wave): 8 for the plain mode, i.e. at most 64 VGPRs; the traversal is partly
+// latency-bound (dependent node fetches) and the extra waves hide more of it than the few
+// spilled registers cost. The seeded and batch modes carry more state per lane (seed clusters,
+// scene tiles) and spill about twice as much at 64 VGPRs: 7 waves (72 VGPRs) is faster for
+// both. A 16-entry stack allows 5 waves by LDS alone. The fp32 node test needs about 100 VGPRs
+// and is left to the compiler (1 = no constraint).
+template <class Node, int S, int MODE>
+constexpr int kWavesPerEu = std::is_same<Node, NodeQ>::value ? (S == 8 ? (MODE == kPlain ? 8 : 7) : 5) : 1;
 
 template <class Node, int S, int B, bool COUNT, int MODE>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<Node, S>))) void k_trace(KArgs a) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<Node, S, MODE>))) void k_trace(KArgs a) {
 	constexpr bool SEEDED = MODE == kSeeded, BATCH = MODE == kBatch;
 	__shared__ uint2 lds[S * B];
 	const Node *nodes = static_cast<const Node *>(a.nodes);
