@@ -363,14 +363,20 @@ def moved(tris, seed, scale):
     return (tris + rng.uniform(-scale, scale, tris.shape)).astype(np.float32)
 
 
-def test_refit_matches_rebuild(gpu_ctx):
+@pytest.mark.parametrize("builder,collapse", [(1, 1), (0, 0)])
+def test_refit_matches_rebuild(gpu_ctx, builder, collapse):
     """Refit to moved geometry: the tree keeps its nodes, every box is recomputed, and the
     records equal the oracle's (and a fresh build's) on the new positions, for a small motion,
-    a large one, and geometry swapped for another scene of the same size."""
+    a large one, and geometry swapped for another scene of the same size; for the default tree
+    (clustering, least-cost collapse) and the batch's (radix tree, greedy collapse)."""
     tris = scenes.scene(0)
     rays = scenes.rays(0, first=200000, count=150000)
     mask = prt.mask_of(prt.ALL_TAGS)
+    gpu_ctx.set_option("builder", builder)
+    gpu_ctx.set_option("collapse", collapse)
     gpu_ctx.set_tris(tris)
+    gpu_ctx.set_option("builder", 1)
+    gpu_ctx.set_option("collapse", 1)
     n0 = check_bvh(gpu_ctx, tris)
     fresh = prt.Context(0)
     for step, new in enumerate((moved(tris, 1, 0.002), moved(tris, 2, 0.2),
