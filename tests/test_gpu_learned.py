@@ -80,6 +80,7 @@ def test_learned_traversal_identical_to_plain(gpu_ctx, room):
     assert after[1] > before[1] + 0.05
     out_plain = torch.zeros(len(test) * size, dtype=torch.uint8, device="cuda")
     out_learn = torch.zeros_like(out_plain)
+    torch.cuda.synchronize()
     for fmt in (1, 0):
         gpu_ctx.set_option("node_format", fmt)
         gpu_ctx.nearest_hits_device(d_test.data_ptr(), len(test), mask, out_plain.data_ptr())
@@ -87,6 +88,21 @@ def test_learned_traversal_identical_to_plain(gpu_ctx, room):
         gpu_ctx.synchronize()
         assert torch.equal(out_plain, out_learn), f"learned differs from plain (node format {fmt})"
     gpu_ctx.set_option("node_format", 1)
+    # every record field, with and without distance culling (without it the seeds only order the
+    # search, and the clusters searched first must still be skipped exactly once)
+    full = prt.mask_of(prt.ALL_TAGS)
+    fsize, _ = prt.layout(full)
+    nf = 262144
+    a = torch.zeros(nf * fsize, dtype=torch.uint8, device="cuda")
+    b = torch.zeros_like(a)
+    torch.cuda.synchronize()  # the library works on its own stream
+    for cull in (1, 0):
+        gpu_ctx.set_option("cull", cull)
+        gpu_ctx.nearest_hits_device(d_test.data_ptr(), nf, full, a.data_ptr())
+        prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), nf, full, b.data_ptr())
+        gpu_ctx.synchronize()
+        assert torch.equal(a, b), f"learned differs from plain (all fields, cull {cull})"
+    gpu_ctx.set_option("cull", 1)
     got = out_learn.cpu().numpy().view(prt.hit_dtype(mask))
     idx = np.random.default_rng(0).choice(len(test), 8192, replace=False)
     want = oracle.trace(room, test[idx], oracle.LOWEST_ID)
