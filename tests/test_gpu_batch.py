@@ -203,6 +203,38 @@ def test_graph_replays_equal_eager(gpu_ctx, batch_workload):
         assert torch.equal(g_out, eager), f"replay {k} differs from the eager run"
 
 
+def test_graph_replay_follows_moved_geometry(gpu_ctx, batch_workload):
+    """The captured graph holds the builds too: triangles moved in place between replays (same
+    counts, as an animation would) are rebuilt and traced by the next replay, equal to an eager
+    run on the moved geometry and to the oracle."""
+    torch = _torch()
+    tris_list, rays_list = batch_workload
+    sub = list(range(2, 64, 8))
+    tl = [tris_list[s] for s in sub]
+    rl = [rays_list[s][:50000] for s in sub]
+    dev = Device(tl, rl, mask=prt.T | prt.PRIMITIVE_ID)
+    batch = prt.Batch(gpu_ctx, dev.tc, dev.rc, dev.mask)
+    run_eager(gpu_ctx, dev, batch)
+    g_out = dev.out()
+    batch.capture(dev.d_tris.data_ptr(), dev.d_rays.data_ptr(), g_out.data_ptr())
+    rng = np.random.default_rng(17)
+    for frame in range(2):
+        moved = [(t.reshape(-1, 3, 3) + rng.uniform(-0.02, 0.02, (len(t), 1, 3))).reshape(-1, 9).astype(np.float32)
+                 for t in tl]
+        dev.d_tris.copy_(torch.from_numpy(np.concatenate(moved)).cuda())
+        torch.cuda.synchronize()
+        print(f"moved frame {frame}", flush=True)
+        batch.replay()
+        batch.check()
+        eager = run_eager(gpu_ctx, dev, batch)
+        assert torch.equal(g_out, eager), f"frame {frame}: replay differs from the eager run"
+        for s in (0, len(sub) - 1):
+            got = dev.records(g_out, s)
+            idx = np.arange(0, len(rl[s]), 13)
+            want = oracle.trace(moved[s], rl[s][idx], oracle.LOWEST_ID)
+            assert_same(got[idx], want, fields=("t", "primitive_id"), what=f"frame {frame} scene {s}")
+
+
 def test_tuner_keeps_results(gpu_ctx, batch_workload):
     torch = _torch()
     tris_list, rays_list = batch_workload
