@@ -395,6 +395,32 @@ def test_refit_matches_rebuild(gpu_ctx, builder, collapse):
     fresh.close()
 
 
+def test_refit_device_matches_host_refit(gpu_ctx):
+    """Refit from device memory (set_tris_device + refit_device) gives the records of the
+    host-buffer refit on a second context, and a wrong triangle count is refused."""
+    import torch
+    tris = scenes.scene(2)
+    new = moved(tris, 3, 0.05)
+    rays = scenes.rays(2, first=3000000, count=200000)
+    mask = prt.mask_of(prt.ALL_TAGS)
+    d_tris = torch.from_numpy(tris).cuda()
+    d_new = torch.from_numpy(new).cuda()
+    torch.cuda.synchronize()
+    gpu_ctx.set_tris_device(d_tris.data_ptr(), len(tris))
+    gpu_ctx.refit_device(d_new.data_ptr(), len(new))
+    got = gpu_ctx.nearest_hits(rays, mask)
+    host = prt.Context(0)
+    host.set_tris(tris)
+    host.refit(new)
+    assert got.tobytes() == host.nearest_hits(rays, mask).tobytes()
+    host.close()
+    idx = np.arange(0, len(rays), 11)
+    assert_same(got[idx], oracle.trace(new, rays[idx]), what="refit_device")
+    with pytest.raises(prt.PrtError) as e:
+        gpu_ctx.refit_device(d_new.data_ptr(), len(new) - 1)
+    assert e.value.code == prt.ERR_ARG
+
+
 def test_refit_errors(gpu_ctx):
     fresh = prt.Context(0)
     tris = scenes.scene(0)[:1000]
