@@ -79,10 +79,22 @@ def build_tools():
     return out
 
 
+def build_cpp_test():
+    """The C++ API test program (tests/cpp/api_test.cpp) against the built library; its rpath is
+    relative ($ORIGIN), so the copy of the tree on another machine finds the library."""
+    out = os.path.join(LIBDIR, "api_test")
+    _run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "api_test.cpp"), "-L", LIBDIR, "-lprt_mi355x",
+          "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_scenes()
     build_hip()
     build_tools()
+    build_cpp_test()
 
 
 if __name__ == "__main__":

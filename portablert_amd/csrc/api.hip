@@ -22,7 +22,8 @@ bool is_gfx950(int dev, std::string *name) {
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
 	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return false;
-	if (name) *name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+	// the marketing name comes from libdrm's table and is empty where that table is missing
+	if (name) *name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ")";
 	return true;
 }
 

@@ -74,6 +74,24 @@ int main() {
 		CHECK(hits > 0);
 		for (auto &r : g)
 			if (r.valid) CHECK(r.t == 1.0f);
+		// refit: the grid lifted by 0.5 keeps the tree, the hits move with it
+		auto *mi = static_cast<MI355XBackend *>(b);
+		Tris lifted = grid;
+		for (auto &t : lifted)
+			for (int v = 2; v < 9; v += 3) t[v] += 0.5f;
+		mi->refit(lifted);
+		auto gl = nearest_hits<filter::t, filter::valid>(rays);
+		for (size_t k = 0; k < rays.size(); ++k) CHECK(gl[k].valid == g[k].valid && (!gl[k].valid || gl[k].t == 1.5f));
+		// move overload, then a call above one chunk of the host pipeline (1,048,576 rays)
+		mi->set_tris(std::move(lifted));
+		std::vector<Ray> many;
+		many.reserve(1500000);
+		for (int k = 0; k < 1500000; ++k) many.push_back(rays[k % rays.size()]);
+		auto gm = nearest_hits<filter::t, filter::primitive_id, filter::valid>(many);
+		CHECK(gm.size() == many.size());
+		int bad = 0;
+		for (size_t k = 0; k < many.size(); ++k) bad += gm[k].valid != gl[k % rays.size()].valid || (gm[k].valid && gm[k].t != 1.5f);
+		CHECK(bad == 0);
 		// error path: non-finite vertex -> std::runtime_error
 		bool threw = false;
 		try {

@@ -235,6 +235,21 @@ def test_calls_on_two_streams(gpu_ctx):
     assert host.tobytes() == d_out[0].cpu().numpy().view(prt.hit_dtype(mask))[:1000].tobytes()
 
 
+def test_cpp_api_on_device():
+    """The header-only C++ layer on the device (tests/cpp/api_test.cpp, built in-tree by
+    build()): registration and selection, set_tris, the free and member nearest_hits with several
+    tag lists and orders, the validation example's known answer, slice(), refit, the move
+    overload of set_tris, a call above one host-pipeline chunk, and the error path."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(prt.LIB), "api_test")
+    assert os.path.exists(exe), "api_test is not built (python portablert_amd/build.py)"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "backend MI355X on" in r.stdout and "all API checks passed" in r.stdout
+
+
 def test_host_entry_point_in_chunks(gpu_ctx):
     """Host calls above one chunk (1,048,576 rays) are pipelined through pinned buffers: a ragged
     count with the full record layout equals the device entry point byte for byte."""
