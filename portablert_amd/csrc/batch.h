@@ -13,7 +13,7 @@
 namespace prt {
 
 constexpr int kBatchBuckets = 4;   // scenes by triangle count: < 4096, < 16384, < 65536, more
-constexpr int kBatchLevels = 48;   // BVH8 levels of the fixed-length (capturable) build
+constexpr int kBatchLevels = kFixedTopLevels + kFixedDfsDepth;  // deepest BVH8 of the capturable build
 
 inline int batch_bucket(uint64_t tris) { return tris < 4096 ? 0 : tris < 16384 ? 1 : tris < 65536 ? 2 : 3; }
 

@@ -4,8 +4,9 @@
 //
 //   * One build over all scenes (build.hip with a SceneSet): the scene index sits above the
 //     Morton code, so one radix sort and one radix tree hold every scene, BVH8 node s is the root
-//     of scene s and leaf records carry scene-local primitive ids. The build runs a fixed number
-//     of BVH8 levels with no host synchronisation, so it can be captured in a graph.
+//     of scene s and leaf records carry scene-local primitive ids. The build runs a fixed sequence
+//     of launches (a few BVH8 level launches, then the rest of the tree depth first) with no
+//     host synchronisation, so it can be captured in a graph.
 //   * One persistent traversal launch over all rays: waves claim work tiles (a run of rays of
 //     one scene) with one atomic each, largest scenes first; each lane starts at its scene's root
 //     with that scene's culling margin.
@@ -193,7 +194,7 @@ int batch_errors(prt_ctx *c, Batch &b) {
 	if (e == hipSuccess) e = trace_errors(b.ts, c->stream, &tf);
 	if (e != hipSuccess) return hip_fail(c, e, "batch error flags");
 	if (bf & 1u) return fail(c, PRT_ERR_INPUT, "batch: a triangle vertex is not finite");
-	if (bf & 4u) return fail(c, PRT_ERR_HIP, "batch: BVH deeper than " + std::to_string(kBatchLevels) + " levels");
+	if (bf & 4u) return fail(c, PRT_ERR_HIP, "batch: a BVH deeper than " + std::to_string(kBatchLevels) + " levels");
 	if (bf) return fail(c, PRT_ERR_HIP, "batch: inconsistent BVH build (flags " + std::to_string(bf) + ")");
 	if (tf) return fail(c, PRT_ERR_HIP, "batch: traversal found a corrupt node or stack bound (flags " +
 	                                       std::to_string(tf) + ")");

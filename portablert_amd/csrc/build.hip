@@ -830,184 +830,217 @@ __global__ __launch_bounds__(kBlock) void k_level_advance(uint32_t *misc) {
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
-	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
+// Collapses BVH8 node i (binary node bin_of[i]): picks its children, allocates its internal
+// children (their binary nodes go to bin_of) and its leaves' triangle slots, and writes both
+// node formats. Returns {first internal child, number of internal children}.
+__device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i) {
 	const BinTree &t = a.t;
 	const uint32_t n_int = t.n - 1;  // binary internal nodes are ids < n_int
-	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
-		const uint32_t b = a.bin_of[i];  // kNone: root of an empty scene, written as an empty node
-		uint32_t el[8];
-		float ar[8];
-		int ne;
-		if (b == kNone) {
-			ne = 0;
-		} else if (b < n_int) {
-			el[0] = t.left[b];
-			el[1] = t.right[b];
-			ne = 2;
-		} else {
-			el[0] = b;
-			ne = 1;
-		}
-		int leaf_size, width;
-		scene_shape(a.cc, b != kNone ? scene_of(t, a.cc, b) : 0u, leaf_size, width);
-		// is_leaf bit k: child k becomes a leaf (by the cost rows, or by its size when greedy)
-		uint32_t is_leaf = 0;
-		if (a.cc.row && ne == 2) {
-			// the width slots split between the two subtrees where that costs least, and each
-			// part again wherever using fewer slots does not cost as little (at most 8 parts)
-			float rl[kCostRow], rr[kCostRow], cx[kCostRow];
-			uint32_t sn[8];
-			int sj[8], sp = 0, k;
-			row_of(t, a.cc, el[0], rl);
-			row_of(t, a.cc, el[1], rr);
-			best_split(rl, rr, width, k);
-			sn[sp] = el[1];
-			sj[sp++] = width - k;
-			sn[sp] = el[0];
-			sj[sp++] = k;
-			ne = 0;
-			while (sp > 0) {
-				--sp;
-				const uint32_t x = sn[sp];
-				int slots = sj[sp];
-				if (x >= n_int) {  // one triangle: a leaf
-					is_leaf |= 1u << ne;
-					el[ne++] = x;
-					continue;
-				}
-				row_of(t, a.cc, x, cx);
-				while (slots > 1 && cx[slots - 1] == cx[slots - 2]) --slots;
-				if (slots == 1) {
-					if (cx[kCostRow - 1] != 0.0f) is_leaf |= 1u << ne;
-					el[ne++] = x;
-					continue;
-				}
-				const uint32_t xl = t.left[x], xr = t.right[x];
-				row_of(t, a.cc, xl, rl);
-				row_of(t, a.cc, xr, rr);
-				best_split(rl, rr, slots, k);
-				sn[sp] = xr;
-				sj[sp++] = slots - k;
-				sn[sp] = xl;
-				sj[sp++] = k;
+	const uint32_t b = a.bin_of[i];  // kNone: root of an empty scene, written as an empty node
+	uint32_t el[8];
+	float ar[8];
+	int ne;
+	if (b == kNone) {
+		ne = 0;
+	} else if (b < n_int) {
+		el[0] = t.left[b];
+		el[1] = t.right[b];
+		ne = 2;
+	} else {
+		el[0] = b;
+		ne = 1;
+	}
+	int leaf_size, width;
+	scene_shape(a.cc, b != kNone ? scene_of(t, a.cc, b) : 0u, leaf_size, width);
+	// is_leaf bit k: child k becomes a leaf (by the cost rows, or by its size when greedy)
+	uint32_t is_leaf = 0;
+	if (a.cc.row && ne == 2) {
+		// the width slots split between the two subtrees where that costs least, and each
+		// part again wherever using fewer slots does not cost as little (at most 8 parts)
+		float rl[kCostRow], rr[kCostRow], cx[kCostRow];
+		uint32_t sn[8];
+		int sj[8], sp = 0, k;
+		row_of(t, a.cc, el[0], rl);
+		row_of(t, a.cc, el[1], rr);
+		best_split(rl, rr, width, k);
+		sn[sp] = el[1];
+		sj[sp++] = width - k;
+		sn[sp] = el[0];
+		sj[sp++] = k;
+		ne = 0;
+		while (sp > 0) {
+			--sp;
+			const uint32_t x = sn[sp];
+			int slots = sj[sp];
+			if (x >= n_int) {  // one triangle: a leaf
+				is_leaf |= 1u << ne;
+				el[ne++] = x;
+				continue;
 			}
-		} else if (ne == 1) {
-			is_leaf = 1u;  // a one-triangle scene
+			row_of(t, a.cc, x, cx);
+			while (slots > 1 && cx[slots - 1] == cx[slots - 2]) --slots;
+			if (slots == 1) {
+				if (cx[kCostRow - 1] != 0.0f) is_leaf |= 1u << ne;
+				el[ne++] = x;
+				continue;
+			}
+			const uint32_t xl = t.left[x], xr = t.right[x];
+			row_of(t, a.cc, xl, rl);
+			row_of(t, a.cc, xr, rr);
+			best_split(rl, rr, slots, k);
+			sn[sp] = xr;
+			sj[sp++] = slots - k;
+			sn[sp] = xl;
+			sj[sp++] = k;
 		}
+	} else if (ne == 1) {
+		is_leaf = 1u;  // a one-triangle scene
+	}
 #pragma unroll
-		for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
-		for (int k = 0; k < ne; ++k)
-			if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
-		while (!a.cc.row && ne < width) {
-			int best = -1;
-			float ba = -1.0f;
+	for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
+	for (int k = 0; k < ne; ++k)
+		if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
+	while (!a.cc.row && ne < width) {
+		int best = -1;
+		float ba = -1.0f;
 #pragma unroll
-			for (int k = 0; k < 8; ++k)
-				if (k < ne && ar[k] > ba) {
-					ba = ar[k];
-					best = k;
-				}
-			if (best < 0) break;
-			const uint32_t x = el[best];
-			const uint32_t l = t.left[x], r = t.right[x];
-			el[best] = l;
-			ar[best] = (l < n_int) ? half_area(t.box[2ull * l], t.box[2ull * l + 1]) : -1.0f;
-			el[ne] = r;
-			ar[ne] = (r < n_int) ? half_area(t.box[2ull * r], t.box[2ull * r + 1]) : -1.0f;
-			++ne;
-		}
-		// classify: greedy collapse makes subtrees of at most leaf_size triangles leaves
-		uint32_t cnt[8];
-		int n_inner = 0;
-		uint32_t n_leaf_tris = 0;
-		for (int k = 0; k < ne; ++k) {
-			cnt[k] = t.count[el[k]];
-			if (!a.cc.row && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
-			if (!((is_leaf >> k) & 1u)) ++n_inner;
-			else n_leaf_tris += cnt[k];
-		}
-		const uint32_t child_base = n_inner ? atomicAdd(a.misc + M_NODES, static_cast<uint32_t>(n_inner)) : 0u;
-		const uint32_t tri_base = n_leaf_tris ? atomicAdd(a.misc + M_TRIS, n_leaf_tris) : 0u;
-		// node box = union of children
-		float4 nlo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
-		float4 nhi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
-		for (int k = 0; k < ne; ++k) {
+		for (int k = 0; k < 8; ++k)
+			if (k < ne && ar[k] > ba) {
+				ba = ar[k];
+				best = k;
+			}
+		if (best < 0) break;
+		const uint32_t x = el[best];
+		const uint32_t l = t.left[x], r = t.right[x];
+		el[best] = l;
+		ar[best] = (l < n_int) ? half_area(t.box[2ull * l], t.box[2ull * l + 1]) : -1.0f;
+		el[ne] = r;
+		ar[ne] = (r < n_int) ? half_area(t.box[2ull * r], t.box[2ull * r + 1]) : -1.0f;
+		++ne;
+	}
+	// classify: greedy collapse makes subtrees of at most leaf_size triangles leaves
+	uint32_t cnt[8];
+	int n_inner = 0;
+	uint32_t n_leaf_tris = 0;
+	for (int k = 0; k < ne; ++k) {
+		cnt[k] = t.count[el[k]];
+		if (!a.cc.row && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
+		if (!((is_leaf >> k) & 1u)) ++n_inner;
+		else n_leaf_tris += cnt[k];
+	}
+	const uint32_t child_base = n_inner ? atomicAdd(a.misc + M_NODES, static_cast<uint32_t>(n_inner)) : 0u;
+	const uint32_t tri_base = n_leaf_tris ? atomicAdd(a.misc + M_TRIS, n_leaf_tris) : 0u;
+	// node box = union of children
+	float4 nlo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+	float4 nhi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
+	for (int k = 0; k < ne; ++k) {
+		const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
+		nlo.x = fminf(nlo.x, lo.x);
+		nlo.y = fminf(nlo.y, lo.y);
+		nlo.z = fminf(nlo.z, lo.z);
+		nhi.x = fmaxf(nhi.x, hi.x);
+		nhi.y = fmaxf(nhi.y, hi.y);
+		nhi.z = fmaxf(nhi.z, hi.z);
+	}
+	NodeQ q;
+	NodeF f;
+	if (ne == 0) nlo = nhi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	const float np[3] = {nlo.x, nlo.y, nlo.z}, nt[3] = {nhi.x, nhi.y, nhi.z};
+	float sc[3];
+#pragma unroll
+	for (int ax = 0; ax < 3; ++ax) {
+		const int e = grid_exp(np[ax], nt[ax]);
+		q.p[ax] = np[ax];
+		q.e[ax] = static_cast<int8_t>(e);
+		sc[ax] = exp2i(e);
+	}
+	q.child_base = f.child_base = child_base;
+	q.tri_base = f.tri_base = tri_base;
+	uint8_t imask = 0;
+	uint32_t inner = 0, toff = 0;
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		uint8_t meta = kMetaEmpty;
+		float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+		float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+		if (k < ne) {
 			const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
-			nlo.x = fminf(nlo.x, lo.x);
-			nlo.y = fminf(nlo.y, lo.y);
-			nlo.z = fminf(nlo.z, lo.z);
-			nhi.x = fmaxf(nhi.x, hi.x);
-			nhi.y = fmaxf(nhi.y, hi.y);
-			nhi.z = fmaxf(nhi.z, hi.z);
+			lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
+			hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
+			if (!((is_leaf >> k) & 1u)) {
+				meta = static_cast<uint8_t>(kMetaInternal | inner);
+				imask |= static_cast<uint8_t>(1u << k);
+				a.bin_of[child_base + inner] = el[k];
+				++inner;
+			} else {
+				meta = meta_leaf(toff, cnt[k]);
+				// the subtree's triangles in depth-first (left to right) order: a range of
+				// sorted positions for the radix tree, any set of them for the clustered tree
+				uint32_t stk[kMaxLeaf + 1];
+				int sp = 0;
+				stk[sp++] = el[k];
+				uint32_t m = 0;
+				while (sp > 0 && m < cnt[k]) {
+					const uint32_t x = stk[--sp];
+					if (x >= n_int) {
+						a.tri_dest[x - n_int] = tri_base + toff + m;
+						++m;
+					} else if (sp + 2 <= kMaxLeaf + 1) {
+						stk[sp++] = t.right[x];
+						stk[sp++] = t.left[x];
+					}
+				}
+				toff += cnt[k];
+			}
 		}
-		NodeQ q;
-		NodeF f;
-		if (ne == 0) nlo = nhi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		const float np[3] = {nlo.x, nlo.y, nlo.z}, nt[3] = {nhi.x, nhi.y, nhi.z};
-		float sc[3];
+		q.meta[k] = f.meta[k] = meta;
 #pragma unroll
 		for (int ax = 0; ax < 3; ++ax) {
-			const int e = grid_exp(np[ax], nt[ax]);
-			q.p[ax] = np[ax];
-			q.e[ax] = static_cast<int8_t>(e);
-			sc[ax] = exp2i(e);
-		}
-		q.child_base = f.child_base = child_base;
-		q.tri_base = f.tri_base = tri_base;
-		uint8_t imask = 0;
-		uint32_t inner = 0, toff = 0;
-#pragma unroll
-		for (int k = 0; k < 8; ++k) {
-			uint8_t meta = kMetaEmpty;
-			float lo3[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-			float hi3[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+			f.lo[ax][k] = lo3[ax];
+			f.hi[ax][k] = hi3[ax];
 			if (k < ne) {
-				const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
-				lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
-				hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
-				if (!((is_leaf >> k) & 1u)) {
-					meta = static_cast<uint8_t>(kMetaInternal | inner);
-					imask |= static_cast<uint8_t>(1u << k);
-					a.bin_of[child_base + inner] = el[k];
-					++inner;
-				} else {
-					meta = meta_leaf(toff, cnt[k]);
-					// the subtree's triangles in depth-first (left to right) order: a range of
-					// sorted positions for the radix tree, any set of them for the clustered tree
-					uint32_t stk[kMaxLeaf + 1];
-					int sp = 0;
-					stk[sp++] = el[k];
-					uint32_t m = 0;
-					while (sp > 0 && m < cnt[k]) {
-						const uint32_t x = stk[--sp];
-						if (x >= n_int) {
-							a.tri_dest[x - n_int] = tri_base + toff + m;
-							++m;
-						} else if (sp + 2 <= kMaxLeaf + 1) {
-							stk[sp++] = t.right[x];
-							stk[sp++] = t.left[x];
-						}
-					}
-					toff += cnt[k];
-				}
-			}
-			q.meta[k] = f.meta[k] = meta;
-#pragma unroll
-			for (int ax = 0; ax < 3; ++ax) {
-				f.lo[ax][k] = lo3[ax];
-				f.hi[ax][k] = hi3[ax];
-				if (k < ne) {
-					quantize(lo3[ax], hi3[ax], np[ax], sc[ax], q.qlo[ax][k], q.qhi[ax][k]);
-				} else {
-					q.qlo[ax][k] = 255;
-					q.qhi[ax][k] = 0;
-				}
+				quantize(lo3[ax], hi3[ax], np[ax], sc[ax], q.qlo[ax][k], q.qhi[ax][k]);
+			} else {
+				q.qlo[ax][k] = 255;
+				q.qhi[ax][k] = 0;
 			}
 		}
-		q.imask = imask;
-		a.nq[i] = q;
-		a.nf[i] = f;
+	}
+	q.imask = imask;
+	a.nq[i] = q;
+	a.nf[i] = f;
+	return make_uint2(child_base, static_cast<uint32_t>(n_inner));
+}
+
+__global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
+	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
+	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) collapse_node(a, i);
+}
+
+// The rest of the tree below the current level, depth first, one frontier node per thread: a
+// fixed-length build (capturable) runs a few level launches for the wide top of the tree and
+// then this one launch instead of a long tail of mostly empty level launches. The stack holds
+// one {first node, nodes left} range per level; a deeper tree sets error bit 2.
+constexpr int kDfsStack = kFixedDfsDepth;
+
+__global__ __launch_bounds__(kBlock) void k_collapse_dfs(CollapseArgs a) {
+	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
+	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
+		uint2 st[kDfsStack];
+		int sp = 0;
+		st[sp++] = make_uint2(i, 1u);
+		while (sp > 0) {
+			const uint32_t node = st[sp - 1].x;
+			if (st[sp - 1].y > 1u) st[sp - 1] = make_uint2(node + 1u, st[sp - 1].y - 1u);
+			else --sp;
+			const uint2 ch = collapse_node(a, node);
+			if (ch.y == 0u) continue;
+			if (sp == kDfsStack) {
+				atomicOr(a.misc + M_ERR, 4u);
+				break;
+			}
+			st[sp++] = ch;
+		}
 	}
 }
 
@@ -1326,7 +1359,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	ca.nq = out.nodes_q.as<NodeQ>();
 	ca.nf = out.nodes_f.as<NodeF>();
 	ca.cc = cc;
-	const int max_levels = fixed ? ss.levels : 128;
+	// a fixed-length build: a few level launches, then the rest of the tree depth first
+	const int max_levels = fixed ? std::min(ss.levels, kFixedTopLevels) : 128;
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
 	for (int level = 0; level < max_levels; ++level) {
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
@@ -1337,6 +1371,11 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 			// done when the level just processed created no new node
 			if (hm[M_NODES] == hm[M_LVL_END]) break;
 		}
+	}
+	if (fixed) {
+		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
+		hipLaunchKernelGGL(k_collapse_dfs, dim3(grid), dim3(kBlock), 0, st, ca);
+		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);  // every node is done
 	}
 	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(), keys,
 	                   off, s.tri_dest.as<uint32_t>(), n, out.tris.as<TriRec>(), misc);

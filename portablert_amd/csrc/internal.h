@@ -81,6 +81,11 @@ struct BuildParams {
 
 constexpr uint32_t kMaxScenes = 65536;
 
+// Fixed-length (graph-capturable) builds: kFixedTopLevels BVH8 level launches, then one launch
+// that collapses the rest depth first with a stack of kFixedDfsDepth levels.
+constexpr int kFixedTopLevels = 6;
+constexpr int kFixedDfsDepth = 64;
+
 // Several scenes in one build (device arrays). BVH8 node s is the root of scene s (an empty node
 // for an empty scene) and leaf triangle records carry scene-local ids.
 struct SceneSet {
@@ -89,10 +94,11 @@ struct SceneSet {
 	const uint32_t *param = nullptr;    // per scene leaf_size | width << 8 (null: BuildParams)
 	float *box = nullptr;               // out: per scene vertex bounds, min xyz max xyz, or null
 	const float *tris = nullptr;        // triangles (null: the scratch's tri_in)
-	int levels = 0;                     // > 0: exactly this many BVH8 levels and no host
-	                                    // synchronisation, so the build can be captured in a
-	                                    // graph once its buffers exist; completion is checked
-	                                    // on the device
+	int levels = 0;                     // > 0: a fixed-length build of at most this many BVH8
+	                                    // levels (min(levels, kFixedTopLevels) level launches,
+	                                    // then the depth-first rest) and no host synchronisation,
+	                                    // so it can be captured in a graph once its buffers
+	                                    // exist; completion is checked on the device
 };
 
 // Builds `out` from n triangles (device, 9 floats each; scratch.tri_in unless the scene set
@@ -102,8 +108,8 @@ struct SceneSet {
 hipError_t build_bvh8(BuildScratch &s, uint64_t n, const BuildParams &p, Bvh8 &out, hipStream_t st,
                       bool *bad_input, double *build_ms, const SceneSet *scenes = nullptr);
 
-// Error bits of the last build on `s`: 1 non-finite vertex, 2 triangle not placed, 4 BVH8 levels
-// exhausted before the build finished. Synchronises st.
+// Error bits of the last build on `s`: 1 non-finite vertex, 2 triangle not placed, 4 the BVH8
+// deeper than a fixed-length build allows. Synchronises st.
 hipError_t build_errors(BuildScratch &s, hipStream_t st, uint32_t *flags);
 
 // New vertex positions (s.tri_in, the same n triangles in input order) for the tree in `b`:
