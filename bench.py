@@ -475,7 +475,7 @@ def main():
     # the traversal kernel alone, timed by the library's HIP events (outside the timed region)
     ctx.set_option("time_kernel", 1)
     lib_ms = []
-    for _ in range(3):
+    for _ in range(5):  # median of 5, after the warm-up and timed steps
         step()
         lib_ms.append(ctx.stat("trace_ms"))
     ctx.set_option("time_kernel", 0)
@@ -509,7 +509,7 @@ def main():
 
     # the host-buffer entry point: PCIe copies of rays in and records out included
     api_ms = []
-    for _ in range(3):
+    for _ in range(5):  # median of 5; the call above was the warm-up
         ctx.nearest_hits(rays, mask)
         api_ms.append(ctx.stat("api_ms"))
     api_ms = float(np.median(api_ms))
