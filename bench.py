@@ -422,8 +422,8 @@ def main():
     ap.add_argument("--no-refit", action="store_true", help="skip the dynamic-geometry (refit) leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (sharding rays or scenes "
+              "across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -7,6 +7,7 @@ import os
 import re
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -171,3 +172,11 @@ def test_batch_tiles_cover_every_ray_once():
     order = [int(s) for k, s in enumerate(scene) if k == 0 or scene[k - 1] != s]
     assert order == sorted(order, key=lambda s: (-int(tri[s]), s))
     assert len(prt.batch_tiles([], [], chunk)) == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why, before any GPU work."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stderr
