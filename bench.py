@@ -43,13 +43,19 @@ def host_info():
             "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
+def host_threads():
+    """Every CPU this process may run on (the reference backend uses hardware_concurrency(),
+    intersect_cpu.hpp:26; on a shared host the process's affinity set is what it can use)."""
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def cpu_baseline(tris, rays, target_s=10.0):
     """The oracle's restatement of the reference CPU backend (binned-SAH BVH2, depth-first
     traversal, static thread split as intersect_cpu.hpp:20-43), timed on this host: a prefix of
     the ray set sized from a short probe so that each of 3 runs takes about target_s / 3 seconds
     (repeated whole passes when one pass is shorter); the median run is reported."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     bvh = oracle.BVH2(tris)
     n = min(len(rays), 1 << 20)  # large enough that thread start-up does not skew the probe
     t0 = time.perf_counter()
@@ -73,14 +79,17 @@ def cpu_baseline(tris, rays, target_s=10.0):
 
 
 def fp64_report(tris, rays, rec):
-    """Error of fp32 hit records against a double-precision re-intersection of the same
-    (ray, triangle) pairs: for t the max relative error and the distance in fp32 ulps, for the
-    barycentrics u and v (in [0, 1]) the absolute error."""
+    """Error of every fp32 hit record against a double-precision re-intersection of the same
+    (ray, triangle) pair: for t the max relative error and the distance in fp32 ulps, for the
+    barycentrics u and v (in [0, 1]) the absolute error, for the hit point p = o + t d (taken
+    from the record, or computed from t as the reference does, bvh.hpp:261-263, when the record
+    has no p) the error relative to |p|_inf and in ulps of its largest component."""
     import oracle
     hit = np.isfinite(rec["t"]) & (rec["primitive_id"] != 0xFFFFFFFF)
     if not hit.any():
         return None
-    t64, u64, v64 = oracle.fp64(tris, rays[hit], rec["primitive_id"][hit])
+    r = rays[hit]
+    t64, u64, v64 = oracle.fp64(tris, r, rec["primitive_id"][hit])
     out = {"hits": int(hit.sum())}
     fields = [("t", t64)] + ([("u", u64), ("v", v64)] if "u" in rec.dtype.names else [])
     for name, ref in fields:
@@ -92,6 +101,17 @@ def fp64_report(tris, rays, rec):
                          "max_ulp": float(ulp.max()), "p99_ulp": float(np.percentile(ulp, 99))}
         else:  # barycentrics lie in [0, 1]: absolute error
             out[name] = {"max_abs": float(err.max()), "p99_abs": float(np.percentile(err, 99))}
+    if "px" in rec.dtype.names:
+        p32 = np.stack([rec["px"][hit], rec["py"][hit], rec["pz"][hit]], 1).astype(np.float64)
+    else:
+        t32 = rec["t"][hit]
+        p32 = (r[:, :3] + t32[:, None] * r[:, 3:]).astype(np.float64)  # float32, two roundings
+    p64 = r[:, :3].astype(np.float64) + t64[:, None] * r[:, 3:].astype(np.float64)
+    scale = np.abs(p64).max(1)
+    err = np.abs(p32 - p64).max(1)
+    ulp = err / np.spacing(scale.astype(np.float32)).astype(np.float64)
+    out["p"] = {"max_rel": float((err / np.maximum(scale, 1e-30)).max()), "max_ulp": float(ulp.max()),
+                "p99_ulp": float(np.percentile(ulp, 99)), "from_record": "px" in rec.dtype.names}
     return out
 
 
@@ -392,9 +412,8 @@ def icosphere_leg(ctx, steps, warmup, stream):
         res[f"{name}_mrays_s"] = round(len(r) / ms / 1e3, 2)
         rec = d_o.cpu().numpy().view(prt.hit_dtype(mask))
         res[f"{name}_negative_t_rate"] = float(np.mean(rec["t"] < 0))
-        idx = np.sort(np.random.default_rng(2).choice(len(r), min(16384, len(r)), replace=False))
         try:
-            res[f"{name}_fp64_error_sample"] = fp64_report(tris, r[idx], rec[idx])
+            res[f"{name}_fp64_error"] = fp64_report(tris, r, rec)
         except Exception as e:  # checker unavailable: report, do not fail the measurement
             log("fp64 check skipped:", e)
     total = len(prim) + len(bounce)
@@ -492,17 +511,18 @@ def main():
         torch.cuda.synchronize()
         log("calibration: 3 x elementwise add over 512 MiB in, 512 MiB out")
 
-    # agreement of the timed output with the oracle on a sample, and the error of that sample
+    # agreement of the timed output with the oracle on every ray, and the error of every hit
     # against a double-precision re-intersection (reported, not timed)
     hits = d_out.cpu().numpy().view(prt.hit_dtype(mask))
     agree, fp64 = None, None
     try:
         import oracle
-        idx = np.sort(np.random.default_rng(1).choice(n, 16384, replace=False))
-        want = oracle.trace(tris, rays[idx], oracle.LOWEST_ID, min(16, os.cpu_count() or 1))
-        agree = float(np.mean((hits["t"][idx].view(np.uint32) == want["t"].view(np.uint32)) &
-                              ((want["t"] == np.inf) | (hits["primitive_id"][idx] == want["primitive_id"]))))
-        fp64 = fp64_report(tris, rays[idx], hits[idx])
+        want = oracle.trace(tris, rays, oracle.LOWEST_ID, host_threads())
+        agree = float(np.mean((hits["t"].view(np.uint32) == want["t"].view(np.uint32)) &
+                              ((want["t"] == np.inf) | (hits["primitive_id"] == want["primitive_id"]))))
+        del want
+        fp64 = fp64_report(tris, rays, hits)
+        log(f"oracle agreement {agree} on {n} rays; fp64 error over {fp64['hits'] if fp64 else 0} hits")
     except Exception as e:  # checker unavailable: report, do not fail the measurement
         log("oracle check skipped:", e)
     neg_t = float(np.mean(hits["t"] < 0))
@@ -568,8 +588,8 @@ def main():
         "traversal_kernel_ms": round(lib_kernel_ms, 4),
         "plain_mrays_s": round(value, 2),
         "learned_mrays_s": None,
-        "oracle_agreement_sample": agree,
-        "fp64_error_sample": fp64,
+        "oracle_agreement": agree,
+        "fp64_error": fp64,
         "negative_t_rate": neg_t,
         "api_boundary_ms": round(api_ms, 3),
         "api_boundary_mrays_s": round(n / api_ms / 1e3, 2),

@@ -83,9 +83,13 @@ class MI355XBackend : public InvokableBackend<MI355XBackend> {
 		    H::has_p::value ? static_cast<int32_t>(offsetof(H, px)) : -1,
 		    H::has_p::value ? static_cast<int32_t>(offsetof(H, py)) : -1,
 		    H::has_p::value ? static_cast<int32_t>(offsetof(H, pz)) : -1};
+		// the field mask from HitRegImpl's has_* members only (hitreg.hpp:29-34), so the header
+		// also compiles against the reference's own hitreg.hpp
+		const uint32_t mask = (H::has_uv::value ? PRT_UV : 0u) | (H::has_t::value ? PRT_T : 0u) |
+		                      (H::has_primitive_id::value ? PRT_PRIMITIVE_ID : 0u) |
+		                      (H::has_p::value ? PRT_P : 0u) | (H::has_valid::value ? PRT_VALID : 0u);
 		const float *r = rays.empty() ? nullptr : rays.data()->origin.data();
-		check(prt_mi355x_nearest_hits(ctx_, r, rays.size(), field_mask<Tags...>(), hits.data(),
-		                              static_cast<uint32_t>(sizeof(H)), off));
+		check(prt_mi355x_nearest_hits(ctx_, r, rays.size(), mask, hits.data(), static_cast<uint32_t>(sizeof(H)), off));
 		return hits;
 	}
 

@@ -38,7 +38,8 @@ enum { PRT_UV = 1, PRT_T = 2, PRT_PRIMITIVE_ID = 4, PRT_P = 8, PRT_VALID = 16 };
 /* Tunables (prt_mi355x_set_option). */
 enum {
 	PRT_OPT_NODE_FORMAT = 1,  /* 0 fp32 BVH8 nodes (208 B), 1 8-bit quantized (80 B, default) */
-	PRT_OPT_CULL = 2,         /* 0 exact: no distance culling; 1 culling with margin (default) */
+	PRT_OPT_CULL = 2,         /* 0 no distance culling; 1 culling with margin (default), boxes holding
+	                             an ill-conditioned triangle never culled */
 	PRT_OPT_LEAF_SIZE = 3,    /* max triangles per leaf, 1..4 (default 3); takes effect at set_tris */
 	PRT_OPT_BLOCK = 4,        /* traversal workgroup size: 64, 128 (default), 256 */
 	PRT_OPT_WAVES_PER_CU = 5, /* persistent waves per CU (default 32) */
@@ -48,7 +49,7 @@ enum {
 	PRT_OPT_COUNT_STATS = 9,  /* 1: count node visits / triangle tests (slower) */
 	PRT_OPT_TIME_KERNEL = 10, /* 1: time the traversal kernel with HIP events (synchronises) */
 	PRT_OPT_ALLOC_LIMIT = 11, /* refuse device allocations above this many bytes (0 = none) */
-	PRT_OPT_MARGIN_REL_LOG2 = 12, /* culling margin relative part 2^-k, k in 4..30 (default 10) */
+	PRT_OPT_MARGIN_REL_LOG2 = 12, /* culling margin relative part 2^-k, k in 4..10 (default 10) */
 	PRT_OPT_BUILDER = 13,     /* binary tree under the BVH8: 0 radix tree over the Morton order, 1
 	                             clustering of Morton-order neighbours (default); at set_tris */
 	PRT_OPT_REFILL = 14,      /* a wave loads new rays once at least v of its lanes are idle, 1..64 (default 8) */
@@ -78,7 +79,9 @@ enum {
 	PRT_STAT_CLUSTERS = 18,    /* BVH8 subtree clusters */
 	PRT_STAT_SEED_TOP1 = 19,   /* last learned_eval: fraction of rays whose top-1 cluster is true */
 	PRT_STAT_SEED_TOP4 = 20,   /* ... whose true cluster is among the top 4 */
-	PRT_STAT_REFIT_MS = 21     /* last refit, device time */
+	PRT_STAT_REFIT_MS = 21,    /* last refit, device time */
+	PRT_STAT_NCULL_TRIS = 22   /* triangles whose boxes are never culled by distance (ill-conditioned
+	                              for the leaf test; counted by set_tris and refit) */
 };
 
 /* Number of usable gfx950 devices and the name of device 0 (name may be NULL). Returns PRT_OK

@@ -25,7 +25,7 @@ OPT = dict(node_format=1, cull=2, leaf_size=3, block=4, waves_per_cu=5, lds_stac
            count_stats=9, time_kernel=10, alloc_limit=11, margin_rel_log2=12, builder=13, refill=14,
            collapse=15)
 STAT = dict(build_ms=1, trace_ms=2, nodes=3, depth=4, node_visits=5, tri_tests=6, leaf_visits=7,
-            node_bytes=8, api_ms=9, n_tris=10, node_phases=11, tri_phases=12, culled_pops=13, refit_ms=21)
+            node_bytes=8, api_ms=9, n_tris=10, node_phases=11, tri_phases=12, culled_pops=13, refit_ms=21, ncull_tris=22)
 
 _lib = None
 

@@ -69,16 +69,6 @@ def build_hip(jobs=4):
     return out
 
 
-def build_tools():
-    """The VALU issue-rate probe (a standalone program; see portablert_amd/tools/valu_rate.hip)."""
-    os.makedirs(LIBDIR, exist_ok=True)
-    out = os.path.join(LIBDIR, "valu_rate")
-    _run([HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          os.path.join(HERE, "tools", "valu_rate.hip"), "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
-    return out
-
-
 def build_cpp_test():
     """The C++ API test program (tests/cpp/api_test.cpp) against the built library; its rpath is
     relative ($ORIGIN), so the copy of the tree on another machine finds the library."""
@@ -93,7 +83,6 @@ def build_cpp_test():
 def build_all():
     build_scenes()
     build_hip()
-    build_tools()
     build_cpp_test()
 
 

@@ -462,7 +462,9 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		c->alloc_limit = v;
 		break;
 	case PRT_OPT_MARGIN_REL_LOG2:
-		if (v < 4 || v > 30) return bad();
+		// the build's never-cull flag covers leaf-test errors up to 2^-10 |t|: a smaller margin
+		// would not be exact
+		if (v < 4 || v > 10) return bad();
 		c->margin_log2 = static_cast<int>(v);
 		apply_margin(c);
 		break;
@@ -507,6 +509,7 @@ int prt_mi355x_get_stat(const prt_ctx *c, int stat, double *v) {
 	switch (stat) {
 	case PRT_STAT_BUILD_MS: *v = c->build_ms; break;
 	case PRT_STAT_REFIT_MS: *v = c->refit_ms; break;
+	case PRT_STAT_NCULL_TRIS: *v = c->bvh.n_ncull == prt::kNcullUnknown ? -1.0 : static_cast<double>(c->bvh.n_ncull); break;
 	case PRT_STAT_TRACE_MS: *v = c->trace_ms; break;
 	case PRT_STAT_NODES: *v = c->bvh.n_nodes; break;
 	case PRT_STAT_DEPTH: *v = c->bvh.depth; break;

@@ -48,6 +48,7 @@ enum : int {
 	M_PNEXT = 8,       // clustering: internal node allocation counter
 	M_PDONE = 9,       // clustering: finished (a step merged nothing)
 	M_PFINAL = 10,     // clustering: which cluster buffer holds the roots
+	M_NCULL = 11,      // triangles flagged by cull_unsafe (never culled by distance)
 	M_SCENE = 12,      // per scene 12 words: ordered centroid min xyz, max xyz, box min, max
 	M_PER_SCENE = 12
 };
@@ -97,7 +98,8 @@ __device__ __forceinline__ void scene_atomics(uint32_t *w, const float c[3], con
 
 constexpr int kPrepBlock = 1024;
 
-// Triangle boxes (the scene index in the w word of the lower corner) and per-scene centroid and
+// Triangle boxes (the scene index in the w word of the lower corner, the never-cull flag in the
+// w word of the upper corner) and per-scene centroid and
 // vertex bounds. The bounds are reduced over the workgroup before the atomics when all of its
 // triangles belong to one scene (one set of 12 atomics per 1024 triangles: same-address
 // atomics serialise), else per lane.
@@ -132,7 +134,10 @@ __global__ __launch_bounds__(kPrepBlock) void k_tri_prep(const float *__restrict
 			smx[k] = hi[k];
 		}
 		tbox[2ull * i] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(scene));
-		tbox[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+		// w of the upper corner: the distance-culling guard's flag (cull_unsafe, common.h)
+		const bool unsafe = cull_unsafe(v);
+		if (unsafe) atomicAdd(misc + M_NCULL, 1u);
+		tbox[2ull * i + 1] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(unsafe ? 1u : 0u));
 	}
 	// the block's first triangle decides the scene the block reduces for
 	const uint32_t first = blockIdx.x * kPrepBlock;
@@ -418,7 +423,7 @@ __device__ __forceinline__ void st_box(float4 *box, uint32_t node, float4 lo, fl
 	__hip_atomic_store(p + 0, pack2(lo.x, lo.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	__hip_atomic_store(p + 1, pack2(lo.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	__hip_atomic_store(p + 2, pack2(hi.x, hi.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	__hip_atomic_store(p + 3, pack2(hi.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(p + 3, pack2(hi.z, hi.w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void ld_box(const float4 *box, uint32_t node, float4 &lo, float4 &hi) {
 	const uint64_t *p = reinterpret_cast<const uint64_t *>(box + 2ull * node);
@@ -428,7 +433,7 @@ __device__ __forceinline__ void ld_box(const float4 *box, uint32_t node, float4 
 	lo = make_float4(__uint_as_float(static_cast<uint32_t>(w[0])), __uint_as_float(static_cast<uint32_t>(w[0] >> 32)),
 	                 __uint_as_float(static_cast<uint32_t>(w[1])), 0.0f);
 	hi = make_float4(__uint_as_float(static_cast<uint32_t>(w[2])), __uint_as_float(static_cast<uint32_t>(w[2] >> 32)),
-	                 __uint_as_float(static_cast<uint32_t>(w[3])), 0.0f);
+	                 __uint_as_float(static_cast<uint32_t>(w[3])), __uint_as_float(static_cast<uint32_t>(w[3] >> 32)));
 }
 // Waits until this lane's earlier stores are complete (device-scope atomic stores are then
 // visible to every CU), so the flag increment that follows publishes them.
@@ -569,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	uint32_t node = t.n - 1 + j;
 	const uint32_t tri = sorted[j];
 	float4 lo = tbox[2ull * tri], hi = tbox[2ull * tri + 1];
-	lo.w = hi.w = 0.0f;
+	lo.w = 0.0f;  // hi.w: the never-cull flag, ORed up the tree with the boxes
 	st_box(t.box, node, lo, hi);
 	t.count[node] = 1;
 	t.first[node] = j;
@@ -589,6 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 		hi.x = fmaxf(hi.x, sh.x);
 		hi.y = fmaxf(hi.y, sh.y);
 		hi.z = fmaxf(hi.z, sh.z);
+		hi.w = __uint_as_float(__float_as_uint(hi.w) | __float_as_uint(sh.w));
 		st_box(t.box, p, lo, hi);
 		if (cc.row) {
 			float cs[kCostRow], cp[kCostRow];
@@ -707,7 +713,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restric
 				const float4 la = t.box[2ull * l], lb = t.box[2ull * l + 1];
 				const float4 ra = t.box[2ull * r], rb = t.box[2ull * r + 1];
 				t.box[2ull * id] = make_float4(fminf(la.x, ra.x), fminf(la.y, ra.y), fminf(la.z, ra.z), la.w);
-				t.box[2ull * id + 1] = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z), 0.0f);
+				t.box[2ull * id + 1] = make_float4(fmaxf(lb.x, rb.x), fmaxf(lb.y, rb.y), fmaxf(lb.z, rb.z),
+				                                   __uint_as_float(__float_as_uint(lb.w) | __float_as_uint(rb.w)));
 				t.left[id] = l;
 				t.right[id] = r;
 				t.count[id] = t.count[l] + t.count[r];
@@ -956,7 +963,7 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 	}
 	q.child_base = f.child_base = child_base;
 	q.tri_base = f.tri_base = tri_base;
-	uint8_t imask = 0;
+	uint8_t ncull = 0;  // slots whose subtree holds a triangle flagged by cull_unsafe
 	uint32_t inner = 0, toff = 0;
 #pragma unroll
 	for (int k = 0; k < 8; ++k) {
@@ -967,9 +974,9 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 			const float4 lo = t.box[2ull * el[k]], hi = t.box[2ull * el[k] + 1];
 			lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
 			hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
+			if (__float_as_uint(hi.w) != 0u) ncull |= static_cast<uint8_t>(1u << k);
 			if (!((is_leaf >> k) & 1u)) {
 				meta = static_cast<uint8_t>(kMetaInternal | inner);
-				imask |= static_cast<uint8_t>(1u << k);
 				a.bin_of[child_base + inner] = el[k];
 				++inner;
 			} else {
@@ -1006,7 +1013,7 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 			}
 		}
 	}
-	q.imask = imask;
+	q.ncull = ncull;
 	a.nq[i] = q;
 	a.nf[i] = f;
 	return make_uint2(child_base, static_cast<uint32_t>(n_inner));
@@ -1122,6 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_tris(const float *__restrict__
 	}
 	tris[j] = r;
 	if (!finite) atomicOr(misc + M_ERR, 1u);
+	if (cull_unsafe(r.v)) atomicAdd(misc + M_NCULL, 1u);
 }
 
 struct RefitArgs {
@@ -1140,6 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(RefitArgs a, uint32_t be
 		NodeQ q = a.nq[i];
 		float nlo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
 		float nhi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+		uint32_t ncull = 0;  // never-cull mask from the moved triangles (cull_unsafe)
 #pragma unroll
 		for (int k = 0; k < 8; ++k) {
 			const uint32_t m = f.meta[k];
@@ -1153,19 +1162,22 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(RefitArgs a, uint32_t be
 					const float4 cl = a.box[2ull * c], ch = a.box[2ull * c + 1];
 					lo[0] = cl.x; lo[1] = cl.y; lo[2] = cl.z;
 					hi[0] = ch.x; hi[1] = ch.y; hi[2] = ch.z;
+					if (__float_as_uint(ch.w) != 0u) ncull |= 1u << k;
 				}
 			} else if (meta_used(m)) {
 				const uint32_t first = f.tri_base + meta_offset(m), cnt = meta_count(m);
 				if (first + cnt > a.n_tris) {
 					atomicOr(a.misc + M_ERR, 2u);
 				} else {
-					for (uint32_t t = first; t < first + cnt; ++t)
+					for (uint32_t t = first; t < first + cnt; ++t) {
 						for (int v = 0; v < 3; ++v)
 #pragma unroll
 							for (int ax = 0; ax < 3; ++ax) {
 								lo[ax] = fminf(lo[ax], a.tris[t].v[3 * v + ax]);
 								hi[ax] = fmaxf(hi[ax], a.tris[t].v[3 * v + ax]);
 							}
+						if (cull_unsafe(a.tris[t].v)) ncull |= 1u << k;
+					}
 				}
 			}
 #pragma unroll
@@ -1177,7 +1189,8 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(RefitArgs a, uint32_t be
 			}
 		}
 		a.box[2ull * i] = make_float4(nlo[0], nlo[1], nlo[2], 0.0f);
-		a.box[2ull * i + 1] = make_float4(nhi[0], nhi[1], nhi[2], 0.0f);
+		a.box[2ull * i + 1] = make_float4(nhi[0], nhi[1], nhi[2], __uint_as_float(ncull != 0u ? 1u : 0u));
+		q.ncull = static_cast<uint8_t>(ncull);
 		float sc[3];
 #pragma unroll
 		for (int ax = 0; ax < 3; ++ax) {
@@ -1228,6 +1241,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	out.levels.clear();
 	out.n_tris = n64;
 	out.n_nodes = 0;
+	out.n_ncull = 0;
 	out.depth = 0;
 	out.leaf_size = static_cast<uint32_t>(p.leaf_size);
 	if (S == 0 || S > kMaxScenes || (S > 1 && !ss.tri_off) || (fixed && build_ms)) return hipErrorInvalidValue;
@@ -1385,6 +1399,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		PRT_TRY(hipGetLastError());
 		out.n_nodes = cap;  // an upper bound: the device holds the count
 		out.depth = static_cast<uint32_t>(ss.levels);
+		out.n_ncull = kNcullUnknown;  // not read back: traversals handle never-cull slots
 		return hipSuccess;
 	}
 	PRT_TRY(hipGetLastError());
@@ -1404,6 +1419,7 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	for (int k = 0; k < 6; ++k) out.scene_box[k] = h2f(hm[M_SCENE + 6 + k]);
 	out.n_nodes = hm[M_NODES];
 	out.depth = hm[M_DEPTH];
+	out.n_ncull = hm[M_NCULL];
 	if (hm[M_TRIS] != n || hm[M_NODES] != hm[M_LVL_END]) return hipErrorUnknown;  // incomplete build
 	return hipSuccess;
 #undef PRT_TRY
@@ -1492,6 +1508,7 @@ hipError_t refit_bvh8(BuildScratch &s, Bvh8 &b, hipStream_t st, bool *bad_input,
 	*bad_input = (err & 1u) != 0;
 	if (*bad_input) return hipSuccess;  // the caller reports the input; the tree is not used
 	if (err & 2u) return hipErrorUnknown;
+	b.n_ncull = static_cast<const uint32_t *>(s.host_misc)[M_NCULL];
 	b.scene_box[0] = root[0].x;
 	b.scene_box[1] = root[0].y;
 	b.scene_box[2] = root[0].z;

@@ -36,11 +36,13 @@ __host__ __device__ inline uint8_t meta_leaf(uint32_t offset, uint32_t count) {
 
 // 8-bit quantized BVH8 node, 80 bytes. Child k's box on axis a is
 // [p_a + qlo_a[k] * 2^e_a, p_a + qhi_a[k] * 2^e_a], evaluated in fp32 as fmaf(q, 2^e, p);
-// the builder verifies each bound rounds outward from the exact child box.
+// the builder verifies each bound rounds outward from the exact child box. Bit k of ncull:
+// child k's subtree holds a triangle flagged by cull_unsafe (never culled by distance); it
+// serves the fp32 node of the same index too.
 struct alignas(16) NodeQ {
 	float p[3];
 	int8_t e[3];
-	uint8_t imask;
+	uint8_t ncull;
 	uint32_t child_base;
 	uint32_t tri_base;
 	uint8_t meta[8];
@@ -145,6 +147,63 @@ __device__ __forceinline__ bool exact_candidate(const float v[9], const float o[
 	const bool box = ref_slab(o, inv, lo, hi);
 	const bool tri = ref_mt(v, o, d, t, u, vv);
 	return box && tri;
+}
+
+// ------------------------------------------------------- distance-culling guard per triangle
+//
+// Distance culling drops a box whose entry distance exceeds t_best + max(m_rel |t_best|, m_abs),
+// so it is exact only if no triangle's Moller-Trumbore t falls further below its own box entry
+// than that margin. With e1, e2 the fp32 edges the leaf test computes, n = e1 x e2 and
+// W_j = |e1_k e2_l| + |e1_l e2_k| ({j, k, l} = {x, y, z}), the standard rounding analysis of the
+// reference arithmetic gives, for a ray (o, d) and s = o - v0,
+//   |det_fp - det| <= 5u (W . |d|),   |num_fp - num| <= 5u (W . |s|)     (u = 2^-24)
+// with det = -(n . d): the relative error of t is driven by rho(d) = (W . |d|) / |n . d|.
+// For a triangle in an axis plane (normal along axis k) every product involving the other two
+// axes is exactly zero, so rho = kappa = W_k / |n_k| for every ray, the numerator's error is
+// relative to s_k (and so to t) with the same factor, and the ray crosses the triangle's flat box
+// exactly at the plane: |t_fp - t| <= (10 kappa + 8) u |t| for any ray. With kappa <= 1024 that
+// is below 2^-10.3 |t|, inside the culling margin m_rel = 2^-10. A tilted triangle's rho also
+// grows as 1 / |cos(angle between d and n)|: kappa_T = rho(n) (its value for the ray along the
+// normal) is its ray-independent part.
+//
+// cull_unsafe() flags a triangle whose kappa_T may exceed 1024 (slivers and degenerate
+// triangles, kappa_T computed in fp32 with fma-exact products and a threshold of 1000, which
+// covers the evaluation's own error), or whose edges or normal lie outside the ranges the bound
+// assumes: edge components 0 or within [2^-30, 2^30] and |n|_inf >= 2^-60. Culling is applied
+// only to rays whose direction components all lie within [2^-30, 2^30] (and |o| < 2^60); then
+// no product of the leaf test overflows, the products of d and the edges stay normal, and a
+// product that underflows (an origin within 2^-100 of a vertex plane) moves t by less than
+// 2^-69 |n_k|^-1 2^-48, below the absolute margin. A flagged triangle's box is never culled:
+// the flag is ORed into every ancestor's child slot.
+__host__ __device__ inline bool cull_unsafe(const float v[9]) {
+#pragma clang fp contract(off)
+	float e1[3], e2[3];
+	for (int k = 0; k < 3; ++k) {
+		e1[k] = v[3 + k] - v[k];
+		e2[k] = v[6 + k] - v[k];
+	}
+	for (int k = 0; k < 3; ++k) {
+		const float a = fabsf(e1[k]), b = fabsf(e2[k]);
+		if (!((a == 0.0f || (a >= 0x1p-30f && a <= 0x1p30f)) && (b == 0.0f || (b >= 0x1p-30f && b <= 0x1p30f))))
+			return true;  // also NaN and infinite edges
+	}
+	float n[3], w[3];
+	for (int j = 0; j < 3; ++j) {
+		const int k = (j + 1) % 3, l = (j + 2) % 3;
+		const float p = e1[l] * e2[k];
+		n[j] = fmaf(e1[k], e2[l], -p);  // e1_k e2_l - e1_l e2_k with one rounding after p's
+		w[j] = fabsf(e1[k] * e2[l]) + fabsf(p);
+	}
+	const float m = fmaxf(fabsf(n[0]), fmaxf(fabsf(n[1]), fabsf(n[2])));
+	if (!(m >= 0x1p-60f)) return true;  // (near) zero area
+	float num = 0.0f, den = 0.0f;
+	for (int j = 0; j < 3; ++j) {
+		const float a = fabsf(n[j]) / m;  // in [0, 1]
+		num += w[j] * a;
+		den += a * a;
+	}
+	// kappa_T = (W . |n|) / (n . n) = num / (m * den); den >= 1
+	return !(num <= 1000.0f * m * den);
 }
 
 // Keep the better of (t, id): smaller t, equal finite t resolved to the lower id. NaN and +inf

@@ -40,12 +40,15 @@ struct DevBuf {
 };
 
 // A BVH8 over one triangle set, resident in HBM.
+constexpr uint32_t kNcullUnknown = 0xFFFFFFFFu;
+
 struct Bvh8 {
 	uint64_t n_tris = 0;
 	uint32_t n_nodes = 0;    // BVH8 nodes (both formats share topology)
 	uint32_t depth = 0;      // BVH8 levels
 	uint32_t leaf_size = 0;  // maximum triangles per leaf child
 	float scene_box[6] = {0, 0, 0, 0, 0, 0};  // min xyz, max xyz of all vertices
+	uint32_t n_ncull = 0;    // triangles flagged by cull_unsafe (kNcullUnknown: not counted)
 	DevBuf nodes_q;          // NodeQ[n_nodes]
 	DevBuf nodes_f;          // NodeF[n_nodes]
 	DevBuf tris;             // TriRec[n_tris], in leaf order

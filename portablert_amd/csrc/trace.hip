@@ -27,6 +27,8 @@ namespace {
 
 struct KArgs {
 	const void *nodes;
+	const NodeQ *nodes_q;   // quantized nodes: their byte 15 is the never-cull mask of either format
+	int any_ncull;          // some triangle is flagged (or the count is unknown)
 	const TriRec *tris;
 	const float *rays;
 	uint8_t *out;
@@ -86,13 +88,18 @@ __device__ __forceinline__ void load_ray(const float *rays, uint64_t idx, float 
 	r.degen = !(__builtin_isfinite(r.inv[0]) && __builtin_isfinite(r.inv[1]) && __builtin_isfinite(r.inv[2]));
 	r.patho = !(__builtin_isfinite(r.o[0]) && __builtin_isfinite(r.o[1]) && __builtin_isfinite(r.o[2])) ||
 	          __builtin_isnan(r.d[0]) || __builtin_isnan(r.d[1]) || __builtin_isnan(r.d[2]);
-	const float big = 0x1p60f;
-	r.fast = fast_scene && !r.degen && !r.patho && fabsf(r.inv[0]) < big && fabsf(r.inv[1]) < big &&
-	         fabsf(r.inv[2]) < big && fabsf(r.o[0]) < big && fabsf(r.o[1]) < big && fabsf(r.o[2]) < big;
-	// margin_abs / |d|, rounded up (1 + 2^-20 covers the reciprocal square root's error); no
-	// absolute margin limit for tiny |d|
+	// |1/d| below 2^30 (every |d_k| above 2^-30), |o| below 2^60 in a scene within 2^60
+	const float big = 0x1p60f, inv_big = 0x1p30f;
+	r.fast = fast_scene && !r.degen && !r.patho && fabsf(r.inv[0]) < inv_big && fabsf(r.inv[1]) < inv_big &&
+	         fabsf(r.inv[2]) < inv_big && fabsf(r.o[0]) < big && fabsf(r.o[1]) < big && fabsf(r.o[2]) < big;
+	// margin_abs / |d|, rounded up (1 + 2^-20 covers the reciprocal square root's error). An
+	// infinite margin turns distance culling off for this ray: culling is exact only for rays
+	// with every direction component within [2^-30, 2^30] (|d|^2 <= 2^60 bounds them above) and
+	// |o| < 2^60 (cull_unsafe in common.h), i.e. fast rays; other rays (zero direction
+	// components, huge or tiny values) traverse without it.
 	const float len2 = r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2];
-	r.abs_margin = len2 >= 0x1p-100f ? margin_abs * (__builtin_amdgcn_rsqf(len2) * (1.0f + 0x1p-20f)) : __builtin_inff();
+	r.abs_margin = (r.fast && len2 <= 0x1p60f) ? margin_abs * (__builtin_amdgcn_rsqf(len2) * (1.0f + 0x1p-20f))
+	                                           : __builtin_inff();
 	r.tb = __builtin_inff();
 	r.ub = r.vb = 0.0f;
 	r.idb = kInvalid;
@@ -218,24 +225,47 @@ struct NodeHits {
 	}
 };
 
-// Tests the children of one node (fast_ok selects the one-fma bound evaluation).
-template <class Node>
-__device__ __forceinline__ void test_children(const Node *nodes, uint32_t idx, const Ray &r, float limit,
-                                              bool fast_ok, NodeHits &h);
+// Key of a child slot whose subtree holds a triangle the build flagged as ill-conditioned
+// (cull_unsafe in common.h): the order-preserving image of -FLT_MAX, which no culling limit
+// rejects, so such a box is never culled (and is visited before its siblings).
+constexpr uint32_t kKeyNoCull = 0x00800000u;
+
+// Tests the children of one node (fast_ok selects the one-fma bound evaluation). Slots whose bit
+// is set in the node's never-cull mask pass whenever the box is hit, at any distance, and take
+// the key kKeyNoCull | slot; the other slots of such a node are tested without the limit and
+// culled only when popped. `ncull_q` is the quantized node array, which holds the mask of
+// both formats (byte 15 of node idx).
+template <class Node, bool NC>
+__device__ __forceinline__ void test_children(const Node *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
+                                              float limit, bool fast_ok, NodeHits &h);
 
 // Byte k of a little-endian dword pair, as a float (one v_cvt_f32_ubyteN when k is constant).
 __device__ __forceinline__ float byte_f(const uint32_t w[2], int k) {
 	return static_cast<float>((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
 }
 
-template <>
-__device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_t idx, const Ray &r, float limit,
-                                                     bool fast_ok, NodeHits &h) {
+// Flagged slots that were hit take the never-cull key (their boxes were tested without the
+// culling limit).
+__device__ __forceinline__ void no_cull_keys(uint32_t key[8], uint32_t ncull) {
+#pragma unroll
+	for (int k = 0; k < 8; ++k)
+		if (((ncull >> k) & 1u) && key[k] != kMiss) key[k] = kKeyNoCull | static_cast<uint32_t>(k);
+}
+
+// NC = false: the scene has no flagged triangle (every mask is 0) and the masks are not read.
+template <class Node, bool NC>
+struct ChildTest;
+
+template <bool NC>
+struct ChildTest<NodeQ, NC> {
+static __device__ __forceinline__ void run(const NodeQ *nodes, const NodeQ * /*ncull_q*/, uint32_t idx, const Ray &r,
+                                           float limit, bool fast_ok, NodeHits &h) {
 	const uint4 *src = reinterpret_cast<const uint4 *>(nodes + idx);
 	const uint4 w0 = src[0], w1 = src[1], w2 = src[2], w3 = src[3], w4 = src[4];
 	const float p[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
 	const float s[3] = {exp2i(static_cast<int8_t>(w0.w & 0xFF)), exp2i(static_cast<int8_t>((w0.w >> 8) & 0xFF)),
 	                    exp2i(static_cast<int8_t>((w0.w >> 16) & 0xFF))};
+	const uint32_t ncull = NC ? (w0.w >> 24) : 0u;  // never-cull mask (the node's byte 15)
 	h.child_base = w1.x;
 	h.lbase = leaf_base(w1.y);
 	const uint32_t meta_lo = w1.z, meta_hi = w1.w;
@@ -271,6 +301,9 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			qf[ax][0] = pos ? qh[ax][0] : ql[ax][0];
 			qf[ax][1] = pos ? qh[ax][1] : ql[ax][1];
 		}
+		// a node with a flagged slot tests every slot without the limit (its unflagged slots are
+		// still culled when popped); the flagged slots' keys are replaced below
+		const float lim = ncull != 0u ? __builtin_inff() : limit;
 #pragma unroll
 		for (int k = 0; k < 8; ++k) {
 			const uint32_t used = ((k < 4 ? unused_lo : unused_hi) >> (8 * (k & 3))) & 0xFFu;
@@ -283,9 +316,10 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			const float tmin = fmaxf(fmaxf(tn[0], tn[1]), tn[2]);
 			const float tmax = fminf(fminf(tf[0], tf[1]), tf[2]);
 			// the fast bounds are finite (no overflow below 2^60), so no clamp in the key
-			const bool hit = used != 0u && tmax >= 0.0f && tmin <= tmax && tmin <= limit;
+			const bool hit = used != 0u && tmax >= 0.0f && tmin <= tmax && tmin <= lim;
 			h.key[k] = hit ? sort_key_finite(tmin, k) : kMiss;
 		}
+		if (ncull != 0u) no_cull_keys(h.key, ncull);
 		sort8(h.key);
 		return;
 	}
@@ -301,18 +335,22 @@ __device__ __forceinline__ void test_children<NodeQ>(const NodeQ *nodes, uint32_
 			lo[ax] = fmaf(static_cast<float>(bl), s[ax], p[ax]);
 			hi[ax] = fmaf(static_cast<float>(bh), s[ax], p[ax]);
 		}
-		const float tmin = child_slab(r, lo, hi, limit);
+		const float tmin = child_slab(r, lo, hi, ncull != 0u ? __builtin_inff() : limit);
 		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		h.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
+	if (ncull != 0u) no_cull_keys(h.key, ncull);
 	sort8(h.key);
 }
+};
 
-template <>
-__device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_t idx, const Ray &r, float limit,
-                                                     bool /*fast_ok*/, NodeHits &hh) {
+template <bool NC>
+struct ChildTest<NodeF, NC> {
+static __device__ __forceinline__ void run(const NodeF *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
+                                           float limit, bool /*fast_ok*/, NodeHits &hh) {
 	const NodeF *n = nodes + idx;
 	const uint4 h = *reinterpret_cast<const uint4 *>(n);
+	const uint32_t ncull = NC ? reinterpret_cast<const uint32_t *>(ncull_q + idx)[3] >> 24 : 0u;
 	hh.child_base = h.x;
 	hh.lbase = leaf_base(h.y);
 	hh.meta = static_cast<uint64_t>(h.z) | (static_cast<uint64_t>(h.w) << 32);
@@ -332,11 +370,19 @@ __device__ __forceinline__ void test_children<NodeF>(const NodeF *nodes, uint32_
 		const uint32_t meta = ((k < 4 ? h.z : h.w) >> (8 * (k & 3))) & 0xFFu;
 		const float l[3] = {lo[0][k], lo[1][k], lo[2][k]};
 		const float u[3] = {hi[0][k], hi[1][k], hi[2][k]};
-		const float tmin = child_slab(r, l, u, limit);
+		const float tmin = child_slab(r, l, u, ncull != 0u ? __builtin_inff() : limit);
 		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		hh.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
+	if (ncull != 0u) no_cull_keys(hh.key, ncull);
 	sort8(hh.key);
+}
+};
+
+template <class Node, bool NC>
+__device__ __forceinline__ void test_children(const Node *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
+                                              float limit, bool fast_ok, NodeHits &h) {
+	ChildTest<Node, NC>::run(nodes, ncull_q, idx, r, limit, fast_ok, h);
 }
 
 // Lane 0's value in every lane, as a wave-uniform (scalar) value; every lane of the wave must be
@@ -387,7 +433,7 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 template <class Node, int S, int MODE>
 constexpr int kWavesPerEu = std::is_same<Node, NodeQ>::value ? (S == 8 ? (MODE == kPlain ? 8 : 7) : 5) : 1;
 
-template <class Node, int S, int B, bool COUNT, int MODE>
+template <class Node, int S, int B, bool COUNT, int MODE, bool NC>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<Node, S, MODE>))) void k_trace(KArgs a) {
 	constexpr bool SEEDED = MODE == kSeeded, BATCH = MODE == kBatch;
 	__shared__ uint2 lds[S * B];
@@ -557,7 +603,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				continue;
 			}
 			NodeHits h;
-			test_children<Node>(nodes, cur, r, cull_limit(r, a), a.cull && r.fast, h);
+			test_children<Node, NC>(nodes, a.nodes_q, cur, r, cull_limit(r, a), a.cull && r.fast, h);
 			if (COUNT) ++c_nodes;
 			cur = h.key[0] != kMiss ? h.entry(h.key[0]) : kInvalid;
 			// clusters searched in the seed phase are not searched again
@@ -594,12 +640,24 @@ __global__ void k_zero(unsigned long long *p, int n) {
 	if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }
 
+template <class Node, int S, int B, int MODE, bool NC>
+void launch_nc(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
+	if (count)
+		hipLaunchKernelGGL((k_trace<Node, S, B, true, MODE, NC>), dim3(grid), dim3(B), 0, st, a);
+	else
+		hipLaunchKernelGGL((k_trace<Node, S, B, false, MODE, NC>), dim3(grid), dim3(B), 0, st, a);
+}
+
+// The batch always reads the never-cull masks (its builds are not read back); the other modes
+// only when the scene has a flagged triangle.
 template <class Node, int S, int B, int MODE>
 void launch_m(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
-	if (count)
-		hipLaunchKernelGGL((k_trace<Node, S, B, true, MODE>), dim3(grid), dim3(B), 0, st, a);
-	else
-		hipLaunchKernelGGL((k_trace<Node, S, B, false, MODE>), dim3(grid), dim3(B), 0, st, a);
+	if constexpr (MODE == kBatch) {
+		launch_nc<Node, S, B, MODE, true>(a, grid, count, st);
+	} else {
+		if (a.any_ncull) launch_nc<Node, S, B, MODE, true>(a, grid, count, st);
+		else launch_nc<Node, S, B, MODE, false>(a, grid, count, st);
+	}
 }
 
 template <class Node, int S, int B>
@@ -659,6 +717,8 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 
 	KArgs a;
 	a.nodes = p.node_format == 0 ? b.nodes_f.ptr : b.nodes_q.ptr;
+	a.nodes_q = b.nodes_q.as<NodeQ>();
+	a.any_ncull = b.n_ncull != 0u ? 1 : 0;
 	a.tris = b.tris.as<TriRec>();
 	a.rays = rays;
 	a.out = static_cast<uint8_t *>(out);

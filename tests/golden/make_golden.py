@@ -96,6 +96,144 @@ def edge_scenes():
     return out
 
 
+def _mt_f32(tri, o, d):
+    """The reference leaf test (core.hpp:27-65) vectorised in float32, every operation rounded on
+    its own (numpy does not fuse): (accepted, t). Used only to search candidates; every case
+    kept is confirmed with the oracle's own oracle.mt."""
+    f32 = np.float32
+    e1, e2 = tri[:, 3:6] - tri[:, 0:3], tri[:, 6:9] - tri[:, 0:3]
+    px = d[:, 1] * e2[:, 2] - d[:, 2] * e2[:, 1]
+    py = d[:, 2] * e2[:, 0] - d[:, 0] * e2[:, 2]
+    pz = d[:, 0] * e2[:, 1] - d[:, 1] * e2[:, 0]
+    det = e1[:, 0] * px + e1[:, 1] * py + e1[:, 2] * pz
+    with np.errstate(all="ignore"):
+        inv = f32(1) / det
+        s = o - tri[:, 0:3]
+        u = (s[:, 0] * px + s[:, 1] * py + s[:, 2] * pz) * inv
+        qx = s[:, 1] * e1[:, 2] - s[:, 2] * e1[:, 1]
+        qy = s[:, 2] * e1[:, 0] - s[:, 0] * e1[:, 2]
+        qz = s[:, 0] * e1[:, 1] - s[:, 1] * e1[:, 0]
+        v = (d[:, 0] * qx + d[:, 1] * qy + d[:, 2] * qz) * inv
+        t = (e2[:, 0] * qx + e2[:, 1] * qy + e2[:, 2] * qz) * inv
+        ok = (det != 0) & ~((u < 0) | (u > 1)) & ~((v < 0) | (u + v > 1))
+    return ok, t
+
+
+def _box_entry_f32(o, d, tri):
+    """Entry distance of the reference slab test (bvh.hpp:195-222) on each triangle's own box,
+    NaN where the test rejects."""
+    lo = np.minimum(np.minimum(tri[:, 0:3], tri[:, 3:6]), tri[:, 6:9])
+    hi = np.maximum(np.maximum(tri[:, 0:3], tri[:, 3:6]), tri[:, 6:9])
+    with np.errstate(all="ignore"):
+        inv = np.float32(1) / d
+        t1, t2 = (lo - o) * inv, (hi - o) * inv
+    mn, mx = np.where(t2 < t1, t2, t1), np.where(t1 < t2, t2, t1)
+    tmin = np.maximum(np.maximum(mn[:, 0], mn[:, 1]), mn[:, 2])
+    tmax = np.minimum(np.minimum(mx[:, 0], mx[:, 1]), mx[:, 2])
+    return np.where(~(tmax < 0) & ~(tmin > tmax), tmin, np.nan)
+
+
+def _sliver_case(rng, centre, tilted):
+    """A sliver triangle near `centre` and a ray for which the reference leaf test accepts it at
+    a t more than 2^-9 (relative) below the entry distance of the sliver's own box: searched
+    with _mt_f32, confirmed with oracle.mt and oracle.slab. Axis-aligned slivers lie in a plane
+    y = const; tilted ones in a random plane."""
+    n = 20000
+    while True:
+        v0 = centre + rng.uniform(-20, 20, (n, 3))
+        if tilted:
+            a = rng.normal(size=(n, 3))
+            a /= np.linalg.norm(a, axis=1, keepdims=True)
+            b = np.cross(a, rng.normal(size=(n, 3)))
+            b /= np.linalg.norm(b, axis=1, keepdims=True)
+        else:
+            ang = rng.uniform(0, 2 * np.pi, n)
+            a = np.stack([np.cos(ang), np.zeros(n), np.sin(ang)], 1)
+            b = np.stack([-np.sin(ang), np.zeros(n), np.cos(ang)], 1)
+        length = 10 ** rng.uniform(-0.5, 1, n)
+        h = length * 10 ** rng.uniform(-6.5, -4, n)
+        frac = rng.uniform(0.2, 0.8, n)
+        tri = np.concatenate([v0, v0 + length[:, None] * a, v0 + (length * frac)[:, None] * a + h[:, None] * b],
+                             1).astype(np.float32)
+        tv = tri.astype(np.float64)
+        b1 = rng.uniform(0.05, 0.9, n)
+        b2 = rng.uniform(0.05, 0.95 - b1)
+        aim = tv[:, 0:3] + b1[:, None] * (tv[:, 3:6] - tv[:, 0:3]) + b2[:, None] * (tv[:, 6:9] - tv[:, 0:3])
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        o = (aim - rng.uniform(8, 30, n)[:, None] * d).astype(np.float32)
+        d = d.astype(np.float32)
+        ok, t = _mt_f32(tri, o, d)
+        te = _box_entry_f32(o, d, tri)
+        with np.errstate(all="ignore"):
+            under = (te - t) / te
+        good = np.flatnonzero(ok & (t > 5) & (under > 2.0 ** -9) & (under < 0.05))
+        for i in good:
+            ray = np.concatenate([o[i], d[i]])
+            hit, tt, _, _ = oracle.mt(tri[i], ray)
+            lo = np.minimum(np.minimum(tri[i, 0:3], tri[i, 3:6]), tri[i, 6:9])
+            hi = np.maximum(np.maximum(tri[i, 0:3], tri[i, 3:6]), tri[i, 6:9])
+            if hit and np.float32(tt) == t[i] and oracle.slab(ray, lo, hi):
+                return tri[i], ray, float(t[i]), float(te[i])
+
+
+def _facing_triangle(rng, ray, t_at, radius, hit):
+    """A small triangle in the plane perpendicular to the ray at distance t_at: hit=True puts
+    the ray inside it (an ordinary, well-conditioned triangle), hit=False outside it but inside
+    its box (the ray crosses the box, the leaf test rejects it)."""
+    o, d = ray[:3].astype(np.float64), ray[3:].astype(np.float64)
+    x = o + t_at * d
+    u1 = np.cross(d, rng.normal(size=3))
+    u1 /= np.linalg.norm(u1)
+    u2 = np.cross(d, u1)
+    for _ in range(10000):
+        if hit:
+            ang = rng.uniform(0, 2 * np.pi) + np.array([0, 2 * np.pi / 3, 4 * np.pi / 3])
+        else:  # three directions within a half-plane: the centre is outside the triangle
+            ang = rng.uniform(0, 2 * np.pi) + np.sort(rng.uniform(0, np.pi * 0.9, 3))
+        vs = [x + radius * (np.cos(g) * u1 + np.sin(g) * u2) for g in ang]
+        tri = np.concatenate(vs).astype(np.float32)
+        h, tt, _, _ = oracle.mt(tri, ray)
+        lo = np.minimum(np.minimum(tri[0:3], tri[3:6]), tri[6:9])
+        hi = np.maximum(np.maximum(tri[0:3], tri[3:6]), tri[6:9])
+        if hit == bool(h) and oracle.slab(ray, lo, hi):
+            return tri
+    raise RuntimeError("no facing triangle found")
+
+
+def sliver_scenes():
+    """Adversarial scenes for distance culling (no reference test covers them): per cluster, a
+    sliver whose Moller-Trumbore t undershoots its own box entry by more than 2^-9, an ordinary
+    competitor the ray hits between the two distances (the reference returns the sliver, which
+    is nearer), and padding so that the competitor is tested before the sliver's box is popped:
+    'gap' one triangle whose box the ray crosses between them, 'cloud' 24 triangles around the
+    competitor, off the ray. Clusters sit 400 units apart so rays meet only their own."""
+    rng = np.random.default_rng(4242)
+    out = {}
+    for name, tilted, cloud, gap in (("sliver_gap", False, 0, True), ("sliver_cloud", False, 24, False),
+                                     ("sliver_tilted", True, 12, True)):
+        tris, rays = [], []
+        for c in range(6):
+            centre = np.array([400.0 * c, 0.0, 0.0])
+            sl, ray, t_c, t_e = _sliver_case(rng, centre, tilted)
+            t_comp = t_c + 0.5 * (t_e - t_c)
+            group = [sl, _facing_triangle(rng, ray, t_comp, 0.02 * t_comp, True)]
+            if gap:
+                group.append(_facing_triangle(rng, ray, t_c + 0.8 * (t_e - t_c), 0.02 * t_comp, False))
+            o, d = ray[:3].astype(np.float64), ray[3:].astype(np.float64)
+            for _ in range(cloud):
+                x = o + t_comp * d + rng.normal(size=3) * 0.1 * t_comp / 20
+                group.append((np.tile(x, 3) + rng.uniform(-0.01, 0.01, 9) * t_comp / 20).astype(np.float32))
+            tris.extend(group)
+            rays.append(ray)
+        tris = np.stack(tris).astype(np.float32)
+        rays = np.stack(rays).astype(np.float32)
+        # shuffle the triangles (ids): the sliver is not always the lowest id of its cluster
+        tris = tris[rng.permutation(len(tris))]
+        out[name] = (tris, rays)
+    return out
+
+
 def workload_samples():
     rng = np.random.default_rng(777)
     res = {}
@@ -139,7 +277,7 @@ def main():
         raise SystemExit("build the reference shim first: make -C oracle")
     if os.path.exists(BUNNY_OBJ):
         bunny_fixture()
-    for name, (tris, rays) in edge_scenes().items():
+    for name, (tris, rays) in list(edge_scenes().items()) + list(sliver_scenes().items()):
         hits = oracle.ref_trace(tris, rays)
         np.savez_compressed(os.path.join(HERE, f"edge_{name}.npz"), tris=tris, rays=rays,
                             hits=hits.view(np.uint8).reshape(-1, 32))

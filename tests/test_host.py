@@ -180,3 +180,155 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "measures one GPU" in r.stderr
+
+
+REF_TREE = "/root/reference"
+
+
+def _integration_snippets():
+    """The fenced code blocks of INTEGRATION.md section 2, in order, without the list indent."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    sec = text.split("## 2. Into the reference tree")[1].split("\n## 3.")[0]
+    blocks, cur = [], None
+    for line in sec.splitlines():
+        body = line[3:] if line.startswith("   ") else line
+        if body.startswith("```"):
+            if cur is None:
+                cur = []
+            else:
+                blocks.append("\n".join(cur) + "\n")
+                cur = None
+        elif cur is not None:
+            cur.append(body)
+    return blocks
+
+
+def _insert(path, anchor, snippet, after=False):
+    src = open(path).read()
+    lines = src.split("\n")
+    idx = [i for i, ln in enumerate(lines) if ln == anchor]
+    assert len(idx) == 1, (path, anchor, len(idx))
+    at = idx[0] + (1 if after else 0)
+    lines[at:at] = snippet.rstrip("\n").split("\n")
+    open(path, "w").write("\n".join(lines))
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_TREE, "src")), reason="reference sources not present")
+def test_reference_tree_integration(tmp_path):
+    """INTEGRATION.md section 2 applied verbatim to a copy of the reference tree: the reference's
+    CPU backend and the MI355X backend are both compiled in (-DUSE_MI355X), the reference's
+    examples build and run, CPU stays the default selection, MI355X is listed among the compiled
+    backends (and among the available ones only where a gfx950 device is visible)."""
+    import re
+    import shutil
+    tree = tmp_path / "ref"
+    for d in ("include", "src"):
+        shutil.copytree(os.path.join(REF_TREE, d), tree / d)
+    for ex in ("triangle", "simple", "backends"):
+        shutil.copytree(os.path.join(REF_TREE, "examples", ex), tree / "examples" / ex)
+    # version.hpp: what the reference's configure_file(version.hpp.in ... @ONLY) generates
+    ver = re.search(r"project\(portableRT VERSION (\d+)\.(\d+)\.(\d+)\)",
+                    open(os.path.join(REF_TREE, "CMakeLists.txt")).read()).groups()
+    tpl = open(os.path.join(REF_TREE, "version.hpp.in")).read()
+    tpl = (tpl.replace("@PROJECT_VERSION@", ".".join(ver)).replace("@PROJECT_VERSION_MAJOR@", ver[0])
+           .replace("@PROJECT_VERSION_MINOR@", ver[1]).replace("@PROJECT_VERSION_PATCH@", ver[2]))
+    (tree / "include" / "portableRT" / "version.hpp").write_text(tpl)
+
+    blocks = _integration_snippets()
+    assert len(blocks) == 6, blocks  # cmake, forward declaration, variant, include, to_variant, include
+    inc = tree / "include" / "portableRT"
+    shutil.copy(os.path.join(ROOT, "include", "prt_mi355x.h"), tree / "include")
+    shutil.copy(os.path.join(ROOT, "include", "portableRT", "intersect_mi355x.hpp"), inc)
+    _insert(inc / "backend.hpp", "class EmbreeCPUBackend;", blocks[1], after=True)
+    _insert(inc / "backend.hpp", "    CPUBackend *>;", blocks[2])
+    _insert(tree / "src" / "backend.cpp", "namespace portableRT {", blocks[3])
+    _insert(tree / "src" / "backend.cpp", "\tif (auto *q = dynamic_cast<CPUBackend *>(backend))", blocks[4])
+    _insert(inc / "portableRT.hpp", '#include "nearest_hits_impl.hpp"', blocks[5])
+    _insert(inc / "nearest_hits_impl.hpp", '#include "backend.hpp"', blocks[5])
+
+    libdir = os.path.dirname(prt.LIB)
+    srcs = [str(tree / "src" / f) for f in ("portableRT.cpp", "intersect_cpu.cpp", "backend.cpp")]
+    outs = {}
+    for ex in ("triangle", "simple", "backends"):
+        exe = tmp_path / ex
+        subprocess.run(["g++", "-std=c++17", "-O1", "-DUSE_MI355X", "-I", str(tree / "include")] + srcs +
+                       [str(tree / "examples" / ex / "main.cpp"), "-L", libdir, "-lprt_mi355x", "-lpthread",
+                        "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+        r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[ex] = r.stdout
+    compiled = outs["backends"].split("compiled backends:")[1].split("available backends:")[0]
+    assert "CPU" in compiled and "MI355X" in compiled, outs["backends"]
+    available = outs["backends"].split("available backends:")[1]
+    assert "CPU" in available
+    assert ("MI355X" in available) == _gpu_present()
+    # the examples test every available backend; CPU (the reference's) gives the known answer
+    assert "Testing CPU" in outs["triangle"] and "Ray t: 1" in outs["triangle"], outs["triangle"]
+    assert "Testing CPU" in outs["simple"] and "Ray 1: 1\nRay 2: 0" in outs["simple"], outs["simple"]
+
+
+HELPERS_CPP = r"""
+#include <cstdio>
+#include <iostream>
+#include <string>
+#include <vector>
+#include <portableRT/core.hpp>
+int main() {
+    portableRT::clear_affinity();
+    for (const auto &s : portableRT::all_tags_combinations()) std::cout << s << "\n";
+    std::cout << "--\n";
+    portableRT::Tri tri;
+    portableRT::Ray ray;
+    while (true) {
+        for (float &x : tri) if (std::scanf("%a", &x) != 1) return 0;
+        for (float &x : ray.origin) if (std::scanf("%a", &x) != 1) return 1;
+        for (float &x : ray.direction) if (std::scanf("%a", &x) != 1) return 1;
+        float t = 0, u = 0, v = 0;
+        const bool hit = portableRT::intersect_tri(tri, ray, t, u, v);
+        std::printf("%d %a %a %a\n", hit ? 1 : 0, hit ? t : 0.0f, hit ? u : 0.0f, hit ? v : 0.0f);
+    }
+}
+"""
+
+
+def _run_helpers(include_dir, tmp_path, tris, rays):
+    src = tmp_path / "helpers.cpp"
+    src.write_text(HELPERS_CPP)
+    exe = tmp_path / ("helpers_" + os.path.basename(os.path.dirname(include_dir)))
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", include_dir, str(src), "-o", str(exe)],
+                   check=True)
+    inp = "\n".join(" ".join(float(x).hex() for x in np.concatenate([t, r])) for t, r in zip(tris, rays))
+    r = subprocess.run([str(exe)], input=inp + "\n", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    names, rows = r.stdout.split("--\n")
+    return names.split(), [ln.split() for ln in rows.strip().splitlines()]
+
+
+def test_core_helpers(tmp_path):
+    """intersect_tri, clear_affinity and all_tags_combinations of the restated core.hpp: the
+    leaf test equals the oracle's restatement of core.hpp:27-65 bit for bit (hits, misses,
+    det == 0, negative t), and the 31 names are TAG_COMBOS' order; where the reference headers
+    are present, the same program built against them prints the same."""
+    import oracle
+    rng = np.random.default_rng(7)
+    n = 3000
+    tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+    rays = np.concatenate([rng.uniform(-2, 2, (n, 3)), rng.normal(size=(n, 3))], 1).astype(np.float32)
+    tris[:5] = [-1, -1, 0, 1, -1, 0, 0, 1, 0]
+    rays[:5] = [[0.1, 0, -1, 0, 0, 1], [0.1, 0, 1, 0, 0, 1], [0.1, 0, -1, 1, 0, 0], [-2, 0, -1, 0, 0, 1],
+                [0.3, 0.2, 1, 0, 0, -1]]
+    names, rows = _run_helpers(os.path.join(ROOT, "include"), tmp_path, tris, rays)
+    assert len(names) == 31 and names[0] == "uv" and names[5] == "uv_t" and names[-1] == "uv_t_primitive_id_p_valid"
+    hits = 0
+    for tri, ray, row in zip(tris, rays, rows):
+        hit, t, u, v = oracle.mt(tri, ray)
+        assert int(row[0]) == int(hit)
+        if hit:
+            hits += 1
+            got = [np.float32(float.fromhex(x)) for x in row[1:]]
+            assert [g.tobytes() for g in got] == [np.float32(x).tobytes() for x in (t, u, v)]
+    assert hits > 50
+    if os.path.isdir(os.path.join(REF_TREE, "include")):
+        ref_names, ref_rows = _run_helpers(os.path.join(REF_TREE, "include"), tmp_path, tris, rays)
+        assert ref_names == names
+        assert ref_rows == rows

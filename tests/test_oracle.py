@@ -187,3 +187,37 @@ def test_bunny_validation_scene():
     assert np.array_equal(low["valid"], got["valid"]) and np.array_equal(low["t"].view(np.uint32), got["t"].view(np.uint32))
     sub = idx[::8]
     assert_same(oracle.brute(tris, rays[sub]), low[sub], what="bunny brute force")
+
+
+def _entry_f32(ray, tri):
+    """Entry distance of the reference slab test (bvh.hpp:195-222) on a triangle's own box."""
+    lo = np.minimum(np.minimum(tri[0:3], tri[3:6]), tri[6:9])
+    hi = np.maximum(np.maximum(tri[0:3], tri[3:6]), tri[6:9])
+    with np.errstate(all="ignore"):
+        inv = np.float32(1) / ray[3:]
+        t1, t2 = (lo - ray[:3]) * inv, (hi - ray[:3]) * inv
+    return float(np.max(np.minimum(t1, t2)))
+
+
+@pytest.mark.parametrize("name", ["sliver_gap", "sliver_cloud", "sliver_tilted"])
+def test_sliver_fixtures_are_adversarial(name):
+    """The distance-culling fixtures hold what they claim, on the oracle alone: for every ray the
+    nearest candidate is a triangle whose Moller-Trumbore t lies more than 2^-9 (relative) below
+    the entry of its own box, and another triangle (the competitor) is a candidate between that
+    t and that box entry, so a traversal that culls by box distance after finding the competitor
+    would return the wrong triangle. The winner agrees with brute force."""
+    g = load_golden("edge_" + name)
+    tris, rays = g["tris"], g["rays"]
+    want = oracle.trace(tris, rays, oracle.LOWEST_ID)
+    assert_same(want, oracle.brute(tris, rays), what=name)
+    for i, ray in enumerate(rays):
+        win = int(want["primitive_id"][i])
+        t_win = float(want["t"][i])
+        entry = _entry_f32(ray, tris[win])
+        assert (entry - t_win) / entry > 2.0 ** -9, (i, t_win, entry)
+        between = []
+        for j, tri in enumerate(tris):
+            hit, t, _, _ = oracle.mt(tri, ray)
+            if j != win and hit and t_win < t < entry:
+                between.append(j)
+        assert between, f"ray {i}: no competitor between {t_win} and {entry}"
