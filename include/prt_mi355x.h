@@ -137,6 +137,20 @@ int prt_mi355x_nearest_hits(prt_ctx *ctx, const float *rays6, uint64_t n, uint32
 int prt_mi355x_nearest_hits_device(prt_ctx *ctx, const float *d_rays6, uint64_t n, uint32_t mask,
                                    void *d_out, uint32_t stride, const int32_t off[8], void *stream);
 
+/* prt_mi355x_nearest_hits_device for callers that cannot allocate device memory (code under
+ * hipGraph stream capture, allocation-free serving loops): no counterpart in the reference. Same
+ * arguments and records; the traversal's work counter and stack spill live in a fixed 128 MiB
+ * device array that each GPU holds from the library's load to the end of the process (its
+ * address is looked up once per device, at prt_mi355x_create), so the call makes no device (or
+ * host) allocation. It launches as many persistent workgroups as that array holds (fewer than
+ * prt_mi355x_nearest_hits_device for deep trees) and fails with PRT_ERR_ARG when not even one
+ * fits (a BVH deeper than about 18,000 levels at the default workgroup of 128 threads). All
+ * calls on one GPU share the array and each launch clears its counter with a kernel: calls that
+ * overlap on one GPU (other streams, other contexts, other threads) race. No timing or statistics options apply; error flags are read
+ * by prt_mi355x_synchronize. */
+int prt_mi355x_nearest_hits_device_noalloc(prt_ctx *ctx, const float *d_rays6, uint64_t n, uint32_t mask,
+                                           void *d_out, uint32_t stride, const int32_t off[8], void *stream);
+
 /* Waits for all work on the context's stream. */
 int prt_mi355x_synchronize(prt_ctx *ctx);
 

@@ -65,6 +65,7 @@ def lib():
         L.prt_mi355x_refit_device.argtypes = [vp, vp, u64]
         L.prt_mi355x_nearest_hits.argtypes = [vp, vp, u64, u32, vp, u32, vp]
         L.prt_mi355x_nearest_hits_device.argtypes = [vp, vp, u64, u32, vp, u32, vp, vp]
+        L.prt_mi355x_nearest_hits_device_noalloc.argtypes = [vp, vp, u64, u32, vp, u32, vp, vp]
         L.prt_mi355x_synchronize.argtypes = [vp]
         L.prt_mi355x_set_option.argtypes = [vp, ci, ctypes.c_int64]
         L.prt_mi355x_get_option.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_int64)]
@@ -187,6 +188,15 @@ class Context:
         self._check(lib().prt_mi355x_nearest_hits_device(self.h, ctypes.c_void_p(rays_ptr), n, mask,
                                                          ctypes.c_void_p(out_ptr), size, _p(offs),
                                                          ctypes.c_void_p(stream or 0)))
+
+    def nearest_hits_device_noalloc(self, rays_ptr, n, mask, out_ptr, stream=None):
+        """nearest_hits_device without any allocation (a fixed device scratch array, shared by
+        every call on the GPU)."""
+        size, off = layout(mask)
+        offs = np.asarray(off, np.int32)
+        self._check(lib().prt_mi355x_nearest_hits_device_noalloc(self.h, ctypes.c_void_p(rays_ptr), n, mask,
+                                                                 ctypes.c_void_p(out_ptr), size, _p(offs),
+                                                                 ctypes.c_void_p(stream or 0)))
 
     def synchronize(self):
         self._check(lib().prt_mi355x_synchronize(self.h))

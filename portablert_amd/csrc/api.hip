@@ -28,6 +28,9 @@ bool is_gfx950(int dev, std::string *name) {
 }
 
 // Checks the caller's record layout against HitReg for the mask.
+// Bytes reserved for error messages at context creation.
+constexpr size_t kErrCapacity = 512;
+
 int check_layout(prt_ctx *c, uint32_t mask, uint32_t stride, const int32_t off[8]) {
 	if (mask > 31u) return fail(c, PRT_ERR_ARG, "field mask has bits above 'valid'");
 	const prt::Layout L = prt::layout_of(mask);
@@ -115,6 +118,10 @@ int prt_mi355x_create(int device, prt_ctx **out) {
 	prt_ctx *c = new prt_ctx();
 	c->device = device;
 	c->name = nm;
+	c->err.reserve(kErrCapacity);  // the no-allocation entry point sets messages without allocating
+	// the fixed traversal scratch's address on this device (prt_mi355x_nearest_hits_device_noalloc);
+	// a failure is reported by that entry point, the others do not use it
+	c->noalloc_setup = prt::noalloc_scratch_setup(device);
 	// default priority (no priority streams anywhere in this library)
 	if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
 		delete c;
@@ -405,11 +412,57 @@ int prt_mi355x_nearest_hits_device(prt_ctx *c, const float *d_rays, uint64_t n, 
 	return trace_common(c, d_rays, n, mask, d_out, st);
 }
 
+// Sets a message without allocating (the context reserved kErrCapacity bytes).
+static int fail_lit(prt_ctx *c, int code, const char *msg) {
+	c->err.assign(msg);
+	return code;
+}
+
+int prt_mi355x_nearest_hits_device_noalloc(prt_ctx *c, const float *d_rays, uint64_t n, uint32_t mask, void *d_out,
+                                           uint32_t stride, const int32_t off[8], void *stream) {
+	if (!c) return PRT_ERR_ARG;
+	c->err.clear();
+	if (mask > 31u) return fail_lit(c, PRT_ERR_ARG, "field mask has bits above 'valid'");
+	const prt::Layout L = prt::layout_of(mask);
+	if (stride != static_cast<uint32_t>(L.size)) return fail_lit(c, PRT_ERR_LAYOUT, "stride != sizeof(HitReg)");
+	if (off)
+		for (int k = 0; k < 8; ++k)
+			if (off[k] != L.off[k]) return fail_lit(c, PRT_ERR_LAYOUT, "field offsets differ from HitReg");
+	if (!c->has_scene) return fail_lit(c, PRT_ERR_STATE, "nearest_hits called before set_tris");
+	if (n == 0) return PRT_OK;
+	if (!d_rays || !d_out) return fail_lit(c, PRT_ERR_ARG, "nearest_hits_device_noalloc: null pointer");
+	if (reinterpret_cast<uintptr_t>(d_rays) % 4 || reinterpret_cast<uintptr_t>(d_out) % 16)
+		return fail_lit(c, PRT_ERR_ARG, "nearest_hits_device_noalloc: rays need 4-byte, records 16-byte alignment");
+	if (hipSetDevice(c->device) != hipSuccess) return fail_lit(c, PRT_ERR_HIP, "hipSetDevice");
+	void *scratch = prt::noalloc_scratch(c->device);
+	if (!scratch) {  // set-up failed at create: once more, on this first call
+		c->noalloc_setup = prt::noalloc_scratch_setup(c->device);
+		scratch = prt::noalloc_scratch(c->device);
+	}
+	if (!scratch) return fail_lit(c, PRT_ERR_HIP, "nearest_hits_device_noalloc: scratch address lookup failed");
+	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+	hipError_t e = prt::order_use(c, st);
+	uint64_t need = 0;
+	if (e == hipSuccess) e = prt::trace_bvh8_noalloc(c->bvh, c->ts, c->tp, d_rays, n, mask, d_out, st, scratch, &need);
+	if (e == hipErrorInvalidValue && need > prt::kNoAllocScratchBytes)
+		return fail_lit(c, PRT_ERR_ARG,
+		                "nearest_hits_device_noalloc: the BVH is too deep for the 128 MiB traversal scratch (one "
+		                "workgroup's stack spill does not fit)");
+	if (e == hipSuccess) e = prt::mark_use(c, st);
+	if (e != hipSuccess) return fail_lit(c, PRT_ERR_HIP, "nearest_hits_device_noalloc: traversal launch failed");
+	c->used_noalloc = true;
+	return PRT_OK;
+}
+
 int prt_mi355x_synchronize(prt_ctx *c) {
 	if (!c) return PRT_ERR_ARG;
 	hipError_t e = hipDeviceSynchronize();  // device entry points may use caller streams
-	uint32_t flags = 0;
+	uint32_t flags = 0, flags_na = 0;
 	if (e == hipSuccess) e = prt::trace_errors(c->ts, c->stream, &flags);
+	if (e == hipSuccess && c->used_noalloc)
+		e = prt::trace_errors_at(c->ts, static_cast<const uint32_t *>(prt::noalloc_scratch(c->device)), c->stream,
+		                         &flags_na);
+	flags |= flags_na;
 	if (e != hipSuccess) return hip_fail(c, e, "synchronize");
 	if (flags) return fail(c, PRT_ERR_HIP, "traversal found a corrupt BVH entry (flags " + std::to_string(flags) + ")");
 	return PRT_OK;

@@ -38,6 +38,10 @@ struct prt_ctx {
 	hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_tr[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
 	prt::DevBuf chunk_err;
 	bool has_scene = false;
+	// traversal without allocation: result of the scratch address look-up at creation, and
+	// whether this context has launched with that scratch (its error flags live there)
+	hipError_t noalloc_setup = hipSuccess;
+	bool used_noalloc = false;
 	int time_kernel = 0;
 	int margin_log2 = 10;
 	int64_t alloc_limit = 0;

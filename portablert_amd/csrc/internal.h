@@ -178,4 +178,19 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
                       uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats,
                       const SeedArgs *seeds = nullptr, const BatchArgs *batch = nullptr);
 
+// Traversal without allocation: the counter and stack spill live in a fixed device array of
+// kNoAllocScratchBytes (trace.hip). noalloc_scratch_setup(device) looks up that array's address
+// on `device` (which must be current) once; noalloc_scratch(device) returns it (null before the
+// set-up). trace_bvh8_noalloc launches as many workgroups as the array's spill part holds and
+// returns hipErrorInvalidValue, with *need_bytes the size one workgroup needs, when not even one
+// fits (a tree deeper than the array allows). No timing and no statistics.
+constexpr uint64_t kNoAllocScratchBytes = uint64_t{128} << 20;
+constexpr uint64_t kNoAllocCounterBytes = 256;
+hipError_t noalloc_scratch_setup(int device);
+void *noalloc_scratch(int device);
+hipError_t trace_bvh8_noalloc(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
+                              uint32_t mask, void *out, hipStream_t st, void *scratch_ptr, uint64_t *need_bytes);
+// Error flags of the last launch that used `counter` (synchronises st).
+hipError_t trace_errors_at(TraceScratch &s, const uint32_t *counter, hipStream_t st, uint32_t *flags);
+
 }  // namespace prt

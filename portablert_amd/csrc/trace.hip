@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -689,15 +690,22 @@ hipError_t launch_b(const KArgs &a, uint32_t grid, int B, int S, bool count, hip
 
 }  // namespace
 
-hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
-                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats, const SeedArgs *seeds,
-                      const BatchArgs *batch) {
 #define PRT_TRY(x)                                                                                 \
 	do {                                                                                           \
 		hipError_t e_ = (x);                                                                       \
 		if (e_ != hipSuccess) return e_;                                                           \
 	} while (0)
-	if (n == 0) return hipSuccess;
+
+namespace {
+
+// Launch shape and stack bound of one traversal.
+struct TracePlan {
+	uint32_t grid = 0;         // workgroups
+	uint32_t max_entries = 0;  // stack bound 7 * depth + 8
+	uint32_t spill_depth = 0;  // stack entries per thread kept in global memory
+};
+
+hipError_t plan_trace(const Bvh8 &b, TraceScratch &s, const TraceParams &p, uint64_t n, TracePlan &pl) {
 	if (s.n_cus == 0) {
 		int dev = 0;
 		PRT_TRY(hipGetDevice(&dev));
@@ -708,13 +716,19 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	uint64_t grid64 = (waves * 64 + B - 1) / B;
 	const uint64_t need_threads = (n + 63) / 64 * 64;
 	grid64 = std::max<uint64_t>(1, std::min<uint64_t>(grid64, (need_threads + B - 1) / B));
-	const uint32_t grid = static_cast<uint32_t>(grid64);
+	pl.grid = static_cast<uint32_t>(grid64);
 	// stack bound: each visited node pushes at most 7 entries, so 7 * depth + 1 suffice
-	const uint32_t max_entries = 7u * std::max(1u, b.depth) + 8u;
-	const uint32_t spill_depth = max_entries > static_cast<uint32_t>(S) ? max_entries - S : 1u;
-	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 8));
-	PRT_TRY(s.spill.ensure(sizeof(uint2) * static_cast<uint64_t>(spill_depth) * grid * B));
+	pl.max_entries = 7u * std::max(1u, b.depth) + 8u;
+	pl.spill_depth = pl.max_entries > static_cast<uint32_t>(S) ? pl.max_entries - S : 1u;
+	return hipSuccess;
+}
 
+// Launches the traversal with the given counter (8 x u64, cleared here by a kernel) and spill
+// array (spill_depth entries per launched thread).
+hipError_t run_trace(const Bvh8 &b, const TraceParams &p, const float *rays, uint64_t n, uint32_t mask, void *out,
+                     hipStream_t st, uint32_t *counter, uint2 *spill, const TracePlan &pl, float *ms,
+                     TraceStats *stats, const SeedArgs *seeds, const BatchArgs *batch) {
+	const int B = p.block, S = p.lds_stack;
 	KArgs a;
 	a.nodes = p.node_format == 0 ? b.nodes_f.ptr : b.nodes_q.ptr;
 	a.nodes_q = b.nodes_q.as<NodeQ>();
@@ -722,15 +736,15 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 	a.tris = b.tris.as<TriRec>();
 	a.rays = rays;
 	a.out = static_cast<uint8_t *>(out);
-	a.counter = s.counter.as<uint32_t>();
-	a.spill = s.spill.as<uint2>();
+	a.counter = counter;
+	a.spill = spill;
 	a.n_rays = n;
 	a.n_nodes = b.n_nodes;
 	a.n_tris = static_cast<uint32_t>(b.n_tris);
-	a.max_entries = max_entries;
+	a.max_entries = pl.max_entries;
 	a.stride = static_cast<uint32_t>(layout_of(mask).size);
 	a.mask = mask;
-	a.spill_depth = spill_depth;
+	a.spill_depth = pl.spill_depth;
 	a.chunk = static_cast<uint32_t>(std::max(64, std::min(1 << 16, p.chunk)));
 	a.leaf_phase = static_cast<uint32_t>(std::max(1, std::min(64, p.leaf_phase)));
 	a.refill = static_cast<uint32_t>(std::max(1, std::min(64, p.refill)));
@@ -756,8 +770,8 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 		PRT_TRY(hipEventRecord(e0, st));
 	}
 	const bool count = p.count_stats != 0;
-	hipError_t e = p.node_format == 0 ? launch_b<NodeF>(a, grid, B, S, count, st)
-	                                  : launch_b<NodeQ>(a, grid, B, S, count, st);
+	hipError_t e = p.node_format == 0 ? launch_b<NodeF>(a, pl.grid, B, S, count, st)
+	                                  : launch_b<NodeQ>(a, pl.grid, B, S, count, st);
 	PRT_TRY(e);
 	if (ms) {
 		PRT_TRY(hipEventRecord(e1, st));
@@ -779,20 +793,92 @@ hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, cons
 		stats->culled_pops = h[7];
 	}
 	return hipSuccess;
-#undef PRT_TRY
 }
 
-hipError_t trace_errors(TraceScratch &s, hipStream_t st, uint32_t *flags) {
+}  // namespace
+
+hipError_t trace_bvh8(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
+                      uint32_t mask, void *out, hipStream_t st, float *ms, TraceStats *stats, const SeedArgs *seeds,
+                      const BatchArgs *batch) {
+	if (n == 0) return hipSuccess;
+	TracePlan pl;
+	PRT_TRY(plan_trace(b, s, p, n, pl));
+	PRT_TRY(s.counter.ensure(sizeof(unsigned long long) * 8));
+	PRT_TRY(s.spill.ensure(sizeof(uint2) * static_cast<uint64_t>(pl.spill_depth) * pl.grid * p.block));
+	return run_trace(b, p, rays, n, mask, out, st, s.counter.as<uint32_t>(), s.spill.as<uint2>(), pl, ms, stats, seeds,
+	                 batch);
+}
+
+// ------------------------------------------------------------ traversal without allocation
+//
+// A device array of fixed size that the library never allocates or frees: the counter in its
+// first 256 bytes, the stack spill after it. Every GPU holds its own copy for the life of the
+// process; its address on each device is looked up once (hipGetSymbolAddress, with that device
+// current) and only that address is passed to kernels or copies. All calls on one GPU share it:
+// calls that overlap on one GPU (other streams, other contexts) race.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device) {
+	if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> g(g_scratch_mu);
+	if (g_scratch_ptr[device]) return hipSuccess;
+	int cur = -1;
+	PRT_TRY(hipGetDevice(&cur));
+	if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+	void *ptr = nullptr;
+	PRT_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+	if (!ptr) return hipErrorInvalidSymbol;
+	g_scratch_ptr[device] = ptr;
+	return hipSuccess;
+}
+
+void *noalloc_scratch(int device) {
+	if (device < 0 || device >= kMaxDevices) return nullptr;
+	std::lock_guard<std::mutex> g(g_scratch_mu);
+	return g_scratch_ptr[device];
+}
+
+hipError_t trace_bvh8_noalloc(const Bvh8 &b, TraceScratch &s, const TraceParams &p, const float *rays, uint64_t n,
+                              uint32_t mask, void *out, hipStream_t st, void *scratch_ptr, uint64_t *need_bytes) {
+	if (n == 0) return hipSuccess;
+	if (!scratch_ptr) return hipErrorInvalidValue;
+	TracePlan pl;
+	PRT_TRY(plan_trace(b, s, p, n, pl));
+	// as many workgroups as the spill part holds (the kernel is persistent: fewer waves trace
+	// the same rays)
+	const uint64_t per_group = sizeof(uint2) * static_cast<uint64_t>(pl.spill_depth) * p.block;
+	const uint64_t fit = (kNoAllocScratchBytes - kNoAllocCounterBytes) / per_group;
+	*need_bytes = kNoAllocCounterBytes + per_group;
+	if (fit == 0) return hipErrorInvalidValue;
+	pl.grid = static_cast<uint32_t>(std::min<uint64_t>(pl.grid, fit));
+	uint8_t *base = static_cast<uint8_t *>(scratch_ptr);
+	return run_trace(b, p, rays, n, mask, out, st, reinterpret_cast<uint32_t *>(base),
+	                 reinterpret_cast<uint2 *>(base + kNoAllocCounterBytes), pl, nullptr, nullptr, nullptr, nullptr);
+}
+
+hipError_t trace_errors_at(TraceScratch &s, const uint32_t *counter, hipStream_t st, uint32_t *flags) {
 	*flags = 0;
-	if (!s.counter.ptr) return hipSuccess;
+	if (!counter) return hipSuccess;
 	if (!s.host_err) {
 		hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&s.host_err), sizeof(uint32_t));
 		if (e != hipSuccess) return e;
 	}
-	hipError_t e = hipMemcpyAsync(s.host_err, s.counter.as<uint32_t>() + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+	hipError_t e = hipMemcpyAsync(s.host_err, counter + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
 	if (e == hipSuccess) e = hipStreamSynchronize(st);
 	if (e == hipSuccess) *flags = *s.host_err;
 	return e;
+}
+
+#undef PRT_TRY
+
+hipError_t trace_errors(TraceScratch &s, hipStream_t st, uint32_t *flags) {
+	return trace_errors_at(s, s.counter.ptr ? s.counter.as<uint32_t>() : nullptr, st, flags);
 }
 
 }  // namespace prt
