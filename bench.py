@@ -422,12 +422,54 @@ def icosphere_leg(ctx, steps, warmup, stream):
     return res
 
 
+def watertight_leg(ctx, steps, warmup, stream):
+    """The optional watertight leaf test (PRT_OPT_LEAF_TEST = 1, no distance culling) beside the
+    reference's Moller-Trumbore on the soup (config 0), the icosphere primaries (config 1) and
+    the room (config 2): traversal time of each, and the fraction of rays whose nearest hit
+    differs (a hit against a miss, or another triangle) or whose t differs."""
+    import torch
+
+    import portablert_amd as prt
+    from portablert_amd import scenes
+    sp = stream.cuda_stream
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    res = {}
+    for w, name in ((0, "soup"), (1, "icosphere_primary"), (2, "room")):
+        tris, rays = scenes.scene(w), scenes.rays(w)
+        n = len(rays)
+        ctx.set_tris(tris)
+        d_r = torch.from_numpy(rays).cuda()
+        outs, ms = {}, {}
+        for lt in (0, 1):
+            ctx.set_option("leaf_test", lt)
+            out = torch.zeros(n * size, dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()
+            ms[lt] = _timed(lambda: ctx.nearest_hits_device(d_r.data_ptr(), n, mask, out.data_ptr(), sp), steps, warmup,
+                            stream)
+            outs[lt] = out.cpu().numpy().view(prt.hit_dtype(mask))
+        ctx.set_option("leaf_test", 0)
+        a, b = outs[0], outs[1]
+        va, vb = a["t"] < np.inf, b["t"] < np.inf
+        differ = (va != vb) | (va & vb & (a["primitive_id"] != b["primitive_id"]))
+        res[name] = {"rays": n, "mt_ms": round(ms[0], 4), "watertight_ms": round(ms[1], 4),
+                     "watertight_mrays_s": round(n / ms[1] / 1e3, 2),
+                     "hit_differs": float(differ.mean()), "mt_hit_wt_miss": float((va & ~vb).mean()),
+                     "wt_hit_mt_miss": float((vb & ~va).mean()),
+                     "t_bits_differ": float((a["t"].view(np.uint32) != b["t"].view(np.uint32)).mean())}
+        log(f"watertight {name}: {ms[1]:.3f} ms vs {ms[0]:.3f} ms; hit differs on {differ.mean():.2e} of rays")
+        del d_r
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=20, help="untimed steps first (the clock settles within about 20)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-watertight", action="store_true")
     ap.add_argument("--format", type=int, default=1, help="0 fp32 nodes, 1 quantized nodes")
     ap.add_argument("--cull", type=int, default=1)
     ap.add_argument("--calib", action="store_true",
@@ -605,6 +647,8 @@ def main():
         out["batch"] = batch_leg(ctx, max(3, args.steps // 4), 1, stream)
     if not args.no_icosphere:
         out["icosphere"] = icosphere_leg(ctx, max(3, args.steps // 4), 1, stream)
+    if not args.no_watertight:
+        out["watertight"] = watertight_leg(ctx, max(3, args.steps // 4), 1, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tris, rays)
     print(json.dumps(out), flush=True)

@@ -20,6 +20,7 @@ REF_LIB = os.path.join(HERE, "_ref", "libprtref.so")
 
 DFS = 0
 LOWEST_ID = 1
+WATERTIGHT = 0x100  # OR into a mode: the watertight leaf test instead of Moller-Trumbore
 
 # FullHitReg layout (reference hitreg.hpp:29-44), 32 bytes
 HIT_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("primitive_id", "<u4"),
@@ -52,6 +53,8 @@ def lib():
         L.orc_brute.argtypes = [vp, sz, vp, sz, ctypes.c_int, vp]
         L.orc_slab.argtypes = [vp, vp, vp]
         L.orc_mt.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_wt.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_brute_mode.argtypes = [vp, sz, vp, sz, ctypes.c_int, ctypes.c_int, vp]
         L.orc_fp64.argtypes = [vp, sz, vp, vp, sz, vp, vp, vp]
         _lib = L
     return _lib
@@ -122,10 +125,12 @@ def trace(tris, rays, mode=LOWEST_ID, threads=0):
     return BVH2(tris).trace(rays, mode, threads)
 
 
-def brute(tris, rays, threads=0):
+def brute(tris, rays, threads=0, mode=LOWEST_ID):
+    """Minimum t over all triangles (own-box slab and leaf test), lowest id; mode may carry
+    WATERTIGHT."""
     tris, rays = _f32(tris, 9), _f32(rays, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
-    lib().orc_brute(_p(tris), len(tris), _p(rays), len(rays), threads, _p(out))
+    lib().orc_brute_mode(_p(tris), len(tris), _p(rays), len(rays), mode, threads, _p(out))
     return out
 
 
@@ -139,6 +144,14 @@ def mt(tri, ray):
     tr, r = _f32(tri, 9), _f32(ray, 6)
     t, u, v = ctypes.c_float(np.nan), ctypes.c_float(np.nan), ctypes.c_float(np.nan)
     hit = lib().orc_mt(_p(tr), _p(r), ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def wt(tri, ray):
+    """(hit, t, u, v) of the watertight leaf test on one pair."""
+    tr, r = _f32(tri, 9), _f32(ray, 6)
+    t, u, v = ctypes.c_float(np.nan), ctypes.c_float(np.nan), ctypes.c_float(np.nan)
+    hit = lib().orc_wt(_p(tr), _p(r), ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
     return bool(hit), t.value, u.value, v.value
 
 

@@ -320,6 +320,93 @@ int orc_mt(const float *tri, const float *r, float *t, float *u, float *v) {
 	return 1;
 }
 
+/* Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection",
+ * JCGT 2(1), 2013), the north star's optional leaf test beside the reference's Moller-Trumbore.
+ * The reference has no such test; the semantics kept from core.hpp:27-65 are: both windings
+ * accepted, t returned without a range check, (u, v) the weights of vertices 1 and 2. Steps:
+ *   kz = axis of the largest |d| (first on ties), kx = kz + 1, ky = kx + 1 (mod 3), kx and ky
+ *   swapped when d[kz] < 0; sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1 / d[kz];
+ *   A, B, C = v0 - o, v1 - o, v2 - o; sheared ax = A[kx] - sx A[kz], ay = A[ky] - sy A[kz]
+ *   (same for B, C); edge functions U = cx by - cy bx, V = ax cy - ay cx, W = bx ay - by ax;
+ *   reject when the signs differ (some < 0 and some > 0); det = U + V + W, reject det == 0;
+ *   T = U sz A[kz] + V sz B[kz] + W sz C[kz]; t = T / det, u = V / det, v = W / det (one
+ *   reciprocal, then products).
+ * An edge function that rounds to exactly 0 is re-evaluated from the exact product errors
+ * (fmaf): then fl(a b) == fl(c e), so a b - c e = (a b - fl(a b)) - (c e - fl(c e)) exactly and
+ * the rounded difference has the exact sign. This replaces the paper's double-precision
+ * fallback; no double is used. The shared edge of two triangles gives exactly opposite values
+ * (the same two products, subtracted in the other order), so a ray through it is accepted by at
+ * least one of them: no ray leaks through a closed mesh. */
+static inline float edge_fn(float a, float b, float c, float e) {
+	const float p = a * b, q = c * e;
+	const float f = p - q;
+	if (f != 0.0f) return f;
+	return fmaf(a, b, -p) - fmaf(c, e, -q);
+}
+
+int orc_wt(const float *tri, const float *r, float *t, float *u, float *v) {
+	const float *o = r, *d = r + 3;
+	int kz = 0;
+	if (fabsf(d[1]) > fabsf(d[kz])) kz = 1;
+	if (fabsf(d[2]) > fabsf(d[kz])) kz = 2;
+	int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+	if (d[kz] < 0.0f) {
+		const int tmp = kx;
+		kx = ky;
+		ky = tmp;
+	}
+	const float sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1.0f / d[kz];
+	const float A[3] = {tri[0] - o[0], tri[1] - o[1], tri[2] - o[2]};
+	const float B[3] = {tri[3] - o[0], tri[4] - o[1], tri[5] - o[2]};
+	const float C[3] = {tri[6] - o[0], tri[7] - o[1], tri[8] - o[2]};
+	const float ax = A[kx] - sx * A[kz], ay = A[ky] - sy * A[kz];
+	const float bx = B[kx] - sx * B[kz], by = B[ky] - sy * B[kz];
+	const float cx = C[kx] - sx * C[kz], cy = C[ky] - sy * C[kz];
+	const float U = edge_fn(cx, by, cy, bx);
+	const float V = edge_fn(ax, cy, ay, cx);
+	const float W = edge_fn(bx, ay, by, ax);
+	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+	const float det = U + V + W;
+	if (det == 0.0f) return 0;
+	const float az = sz * A[kz], bz = sz * B[kz], cz = sz * C[kz];
+	const float T = U * az + V * bz + W * cz;
+	const float rcp = 1.0f / det;
+	*t = T * rcp;
+	*u = V * rcp;
+	*v = W * rcp;
+	return 1;
+}
+
+/* Entry and exit distances of the reference slab test (bvh.hpp:195-222, the same operations). */
+static void slab_interval(const float *r, const float *bmin, const float *bmax, float *tmin, float *tmax) {
+	const float df[3] = {1.0f / r[3], 1.0f / r[4], 1.0f / r[5]};
+	const float t1 = (bmin[0] - r[0]) * df[0];
+	const float t2 = (bmax[0] - r[0]) * df[0];
+	const float t3 = (bmin[1] - r[1]) * df[1];
+	const float t4 = (bmax[1] - r[1]) * df[1];
+	const float t5 = (bmin[2] - r[2]) * df[2];
+	const float t6 = (bmax[2] - r[2]) * df[2];
+	*tmin = smax(smax(smin(t1, t2), smin(t3, t4)), smin(t5, t6));
+	*tmax = smin(smin(smax(t1, t2), smax(t3, t4)), smax(t5, t6));
+}
+
+static float slab_tmax(const float *r, const float *bmin, const float *bmax) {
+	float lo, hi;
+	slab_interval(r, bmin, bmax, &lo, &hi);
+	return hi;
+}
+
+/* Candidate test of a triangle in a trace mode (the leaf of bvh.hpp:238-246). The reference:
+ * its own box passes the slab test, then Moller-Trumbore. ORC_WATERTIGHT: the watertight test,
+ * and the own box not entirely behind the origin (the slab test's tmax >= 0 alone; its
+ * tmin <= tmax part can fail by an ulp at a box corner a ray grazes, which would let rays through
+ * shared vertices of a closed mesh). */
+static inline int candidate(int mode, const float *tri, const float *r, float *t, float *u, float *v) {
+	const aabb_t box = make_aabb(tri);
+	if (mode & ORC_WATERTIGHT) return !(slab_tmax(r, box.lo, box.hi) < 0.0f) && orc_wt(tri, r, t, u, v);
+	return orc_slab(r, box.lo, box.hi) && orc_mt(tri, r, t, u, v);
+}
+
 /* Result write-out of nearest_tri (bvh.hpp:259-263). */
 static void finish(const float *r, float t_near, float u, float v, uint32_t id, orc_hit *h) {
 	const int valid = t_near < INFINITY;
@@ -344,12 +431,23 @@ static void nearest_tri(const orc_bvh *b, const float *r, int mode, orc_hit *h) 
 	uint32_t bid = 0xFFFFFFFFu;
 	while (sp > 0) {
 		const node_t *n = &b->nodes[stack[--sp]];
-		if (!orc_slab(r, n->box.lo, n->box.hi)) continue;
+		if (mode & ORC_WATERTIGHT) {
+			/* the candidate test decides; node boxes only need to be conservative: the slab
+			 * interval widened by 2^-20 of its end points, so that a ray through a box corner
+			 * (a vertex shared by several triangles) is never rejected by rounding */
+			float lo_t, hi_t;
+			slab_interval(r, n->box.lo, n->box.hi, &lo_t, &hi_t);
+			const float slack = 0x1p-20f * (fabsf(lo_t) + fabsf(hi_t));
+			if (hi_t < -slack || lo_t > hi_t + slack) continue;
+		} else if (!orc_slab(r, n->box.lo, n->box.hi)) {
+			continue;
+		}
 		if (n->is_leaf) {
 			if (n->tri == 0xFFFFFFFFu) continue;
 			float t, u, v;
-			if (orc_mt(b->tris + 9 * (size_t)n->tri, r, &t, &u, &v)) {
-				const int take = (t < t_near) || (mode == ORC_LOWEST_ID && t == t_near &&
+			if ((mode & ORC_WATERTIGHT) ? candidate(mode, b->tris + 9 * (size_t)n->tri, r, &t, &u, &v)
+			                            : orc_mt(b->tris + 9 * (size_t)n->tri, r, &t, &u, &v)) {
+				const int take = (t < t_near) || ((mode & 0xFF) == ORC_LOWEST_ID && t == t_near &&
 				                                   t_near < INFINITY && n->tri < bid);
 				if (take) {
 					t_near = t;
@@ -367,15 +465,13 @@ static void nearest_tri(const orc_bvh *b, const float *r, int mode, orc_hit *h) 
 	finish(r, t_near, bu, bv, bid, h);
 }
 
-static void brute_one(const float *tris, size_t n_tris, const float *r, orc_hit *h) {
+static void brute_one(const float *tris, size_t n_tris, const float *r, int mode, orc_hit *h) {
 	float t_near = INFINITY, bu = 0.0f, bv = 0.0f;
 	uint32_t bid = 0xFFFFFFFFu;
 	for (size_t i = 0; i < n_tris; ++i) {
 		const float *tri = tris + 9 * i;
-		const aabb_t box = make_aabb(tri);
-		if (!orc_slab(r, box.lo, box.hi)) continue;
 		float t, u, v;
-		if (orc_mt(tri, r, &t, &u, &v) && t < t_near) {
+		if (candidate(mode, tri, r, &t, &u, &v) && t < t_near) {
 			t_near = t;
 			bu = u;
 			bv = v;
@@ -403,7 +499,7 @@ static void *run_job(void *arg) {
 	for (size_t i = 0; i < j->count; ++i) {
 		const size_t r = j->first + i;
 		if (j->brute)
-			brute_one(j->tris, j->n_tris, j->rays + 6 * r, &j->out[r]);
+			brute_one(j->tris, j->n_tris, j->rays + 6 * r, j->mode, &j->out[r]);
 		else
 			nearest_tri(j->b, j->rays + 6 * r, j->mode, &j->out[r]);
 	}
@@ -464,6 +560,11 @@ int orc_trace(const orc_bvh *b, const float *rays, size_t n, int tie_mode, int t
 int orc_brute(const float *tris, size_t n_tris, const float *rays, size_t n, int threads,
               orc_hit *out) {
 	return run(NULL, tris, n_tris, rays, n, ORC_LOWEST_ID, 1, threads, out);
+}
+
+int orc_brute_mode(const float *tris, size_t n_tris, const float *rays, size_t n, int mode,
+                   int threads, orc_hit *out) {
+	return run(NULL, tris, n_tris, rays, n, ORC_LOWEST_ID | (mode & ORC_WATERTIGHT), 1, threads, out);
 }
 
 int orc_fp64(const float *tris, size_t n_tris, const float *rays, const uint32_t *ids, size_t n,

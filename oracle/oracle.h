@@ -10,6 +10,7 @@
  *   orc_trace        traversal + exact leaf test     bvh.hpp:195-265, core.hpp:27-65
  *                    threading as CPUBackend         intersect_cpu.hpp:12-43
  *   orc_brute        O(N*M) predicate oracle (slab(aabb(T)) and Moller-Trumbore, lowest id)
+ *   orc_wt           watertight leaf test (Woop, Benthin, Wald 2013), the optional GPU mode
  *   orc_fp64         double-precision re-intersection of a given (ray, triangle) pair
  *
  * Pinning (what fixes each function independently of this restatement) is listed in
@@ -24,10 +25,11 @@
 extern "C" {
 #endif
 
-/* Tie rule at equal t. */
+/* Tie rule at equal t, optionally ORed with ORC_WATERTIGHT (leaf test). */
 enum {
 	ORC_DFS = 0,       /* reference: first found in depth-first order (strict <, bvh.hpp:247) */
 	ORC_LOWEST_ID = 1, /* same traversal, equal t resolved to the lowest primitive id */
+	ORC_WATERTIGHT = 0x100, /* leaf test: orc_wt instead of the reference's Moller-Trumbore */
 };
 
 /* Full hit record in the reference FullHitReg layout (hitreg.hpp:29-44): 32 bytes. For misses
@@ -63,6 +65,12 @@ int orc_brute(const float *tris, size_t n_tris, const float *rays, size_t n, int
 /* The two predicates on one (ray, triangle) pair, exactly as the reference evaluates them. */
 int orc_slab(const float *ray6, const float *bmin3, const float *bmax3);
 int orc_mt(const float *tri9, const float *ray6, float *t, float *u, float *v);
+/* Watertight test (Woop, Benthin and Wald 2013, fp32 with an fma-exact zero fallback); no
+ * reference counterpart: the optional leaf test of the GPU backend. */
+int orc_wt(const float *tri9, const float *ray6, float *t, float *u, float *v);
+/* orc_brute with a leaf-test choice (mode & ORC_WATERTIGHT); minimum t, lowest id. */
+int orc_brute_mode(const float *tris, size_t n_tris, const float *rays, size_t n, int mode,
+                   int threads, orc_hit *out);
 
 /* Moller-Trumbore in double on the fp32 inputs for pair (ray i, triangle ids[i]); ids equal to
  * 0xFFFFFFFF give NaN. Writes t, u, v in double. */

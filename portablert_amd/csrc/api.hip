@@ -521,6 +521,10 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		c->margin_log2 = static_cast<int>(v);
 		apply_margin(c);
 		break;
+	case PRT_OPT_LEAF_TEST:
+		if (v != 0 && v != 1) return bad();
+		c->tp.leaf_test = static_cast<int>(v);
+		break;
 	case PRT_OPT_BUILDER:
 		if (v != 0 && v != 1) return bad();
 		c->bp.builder = static_cast<int>(v);
@@ -552,6 +556,7 @@ int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
 	case PRT_OPT_MARGIN_REL_LOG2: *v = c->margin_log2; break;
 	case PRT_OPT_BUILDER: *v = c->bp.builder; break;
 	case PRT_OPT_COLLAPSE: *v = c->bp.collapse; break;
+	case PRT_OPT_LEAF_TEST: *v = c->tp.leaf_test; break;
 	default: return PRT_ERR_ARG;
 	}
 	return PRT_OK;

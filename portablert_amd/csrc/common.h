@@ -149,6 +149,86 @@ __device__ __forceinline__ bool exact_candidate(const float v[9], const float o[
 	return box && tri;
 }
 
+// Watertight leaf test (Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT
+// 2(1), 2013), the optional mode beside the reference's Moller-Trumbore. Semantics kept from
+// core.hpp:27-65: both windings, no t range check, (u, v) the weights of vertices 1 and 2. The
+// ray is sheared so that it runs along +z of the axes (kx, ky, kz), kz the largest |d|
+// component; the 2-D edge functions U, V, W of the sheared vertices decide the hit by sign, and
+// one that rounds to exactly 0 is re-evaluated from the fma-exact product errors, which gives
+// its exact sign (fl(ab) == fl(ce) there, so ab - ce = (ab - fl(ab)) - (ce - fl(ce))): a ray
+// through a shared edge is accepted by at least one of the two triangles. fp32 only.
+__device__ __forceinline__ float wt_edge(float a, float b, float c, float e) {
+#pragma clang fp contract(off)
+	const float p = a * b, q = c * e;
+	const float f = p - q;
+	return f != 0.0f ? f : fmaf(a, b, -p) - fmaf(c, e, -q);
+}
+
+__device__ __forceinline__ float sel3(int k, float a0, float a1, float a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
+
+__device__ __forceinline__ bool wt_test(const float v[9], const float o[3], const float d[3], float &t, float &u,
+                                        float &vv) {
+#pragma clang fp contract(off)
+	int kz = 0;
+	if (fabsf(d[1]) > fabsf(d[kz])) kz = 1;
+	if (fabsf(d[2]) > fabsf(sel3(kz, d[0], d[1], d[2]))) kz = 2;
+	int kx = kz == 2 ? 0 : kz + 1;
+	int ky = kx == 2 ? 0 : kx + 1;
+	const float dz = sel3(kz, d[0], d[1], d[2]);
+	if (dz < 0.0f) {
+		const int tmp = kx;
+		kx = ky;
+		ky = tmp;
+	}
+	const float sx = sel3(kx, d[0], d[1], d[2]) / dz, sy = sel3(ky, d[0], d[1], d[2]) / dz, sz = 1.0f / dz;
+	float rel[3][3];  // vertex - origin
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int k = 0; k < 3; ++k) rel[i][k] = v[3 * i + k] - o[k];
+	float x[3], y[3], z[3];
+#pragma unroll
+	for (int i = 0; i < 3; ++i) {
+		const float rx = sel3(kx, rel[i][0], rel[i][1], rel[i][2]);
+		const float ry = sel3(ky, rel[i][0], rel[i][1], rel[i][2]);
+		z[i] = sel3(kz, rel[i][0], rel[i][1], rel[i][2]);
+		x[i] = rx - sx * z[i];
+		y[i] = ry - sy * z[i];
+	}
+	const float U = wt_edge(x[2], y[1], y[2], x[1]);
+	const float V = wt_edge(x[0], y[2], y[0], x[2]);
+	const float W = wt_edge(x[1], y[0], y[1], x[0]);
+	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+	const float det = U + V + W;
+	if (det == 0.0f) return false;
+	const float T = U * (sz * z[0]) + V * (sz * z[1]) + W * (sz * z[2]);
+	const float rcp = 1.0f / det;
+	t = T * rcp;
+	u = V * rcp;
+	vv = W * rcp;
+	return true;
+}
+
+// Candidate test of the watertight mode: the watertight test, and the triangle's exact box not
+// entirely behind the origin (the reference slab test's tmax >= 0, bvh.hpp:213, with its
+// operations; its tmin <= tmax part is left out because it can fail by an ulp at a box corner
+// that a ray through a shared vertex grazes).
+__device__ __forceinline__ bool wt_candidate(const float v[9], const float o[3], const float d[3],
+                                             const float inv[3], float &t, float &u, float &vv) {
+#pragma clang fp contract(off)
+	float tmax = __builtin_inff();
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		const float lo = smin(v[k], smin(v[3 + k], v[6 + k]));
+		const float hi = smax(v[k], smax(v[3 + k], v[6 + k]));
+		const float t1 = (lo - o[k]) * inv[k], t2 = (hi - o[k]) * inv[k];
+		tmax = k == 0 ? smax(t1, t2) : smin(tmax, smax(t1, t2));
+	}
+	const bool box = !(tmax < 0.0f);
+	const bool tri = wt_test(v, o, d, t, u, vv);
+	return box && tri;
+}
+
 // ------------------------------------------------------- distance-culling guard per triangle
 //
 // Distance culling drops a box whose entry distance exceeds t_best + max(m_rel |t_best|, m_abs),

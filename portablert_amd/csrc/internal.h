@@ -131,6 +131,7 @@ struct TraceParams {
 	int leaf_phase = 24;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int refill = 8;           // load new rays once at least this many lanes of a wave are idle
 	int count_stats = 0;      // count node visits and triangle tests
+	int leaf_test = 0;        // 0 Moller-Trumbore (reference), 1 watertight (no distance culling)
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)
 };

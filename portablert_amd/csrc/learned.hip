@@ -276,6 +276,8 @@ int prt_mi355x_nearest_hits_learned_device(prt_ctx *c, const float *d_rays, uint
 	if (!d_rays || !d_out) return fail(c, PRT_ERR_ARG, "null pointer");
 	if (reinterpret_cast<uintptr_t>(d_rays) % 4 || reinterpret_cast<uintptr_t>(d_out) % 16)
 		return fail(c, PRT_ERR_ARG, "rays need 4-byte, records 16-byte alignment");
+	if (c->tp.leaf_test != 0)
+		return fail(c, PRT_ERR_STATE, "learned seeding uses the Moller-Trumbore leaf test (set leaf_test to 0)");
 	if (hipSetDevice(c->device) != hipSuccess) return fail(c, PRT_ERR_HIP, "hipSetDevice");
 	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;

@@ -30,6 +30,7 @@ struct KArgs {
 	const void *nodes;
 	const NodeQ *nodes_q;   // quantized nodes: their byte 15 is the never-cull mask of either format
 	int any_ncull;          // some triangle is flagged (or the count is unknown)
+	int leaf_test;          // 0 Moller-Trumbore (the reference's), 1 watertight
 	const TriRec *tris;
 	const float *rays;
 	uint8_t *out;
@@ -61,7 +62,8 @@ struct KArgs {
 	const float2 *scene_info;
 };
 
-enum : int { kPlain = 0, kSeeded = 1, kBatch = 2 };
+// kWatertight: the plain traversal with the watertight leaf test (wt_candidate), no culling
+enum : int { kPlain = 0, kSeeded = 1, kBatch = 2, kWatertight = 3 };
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
@@ -113,7 +115,10 @@ __device__ __forceinline__ float cull_limit(const Ray &r, const KArgs &a) {
 }
 
 // Conservative slab test of one child box [lo, hi]. Returns the entry distance (tmin), or NaN
-// when the box is rejected.
+// when the box is rejected. WIDE (the watertight mode): the interval is widened by 2^-20 of its
+// end points, so that a ray through a box corner (a vertex shared by several triangles) is not
+// rejected by the rounding of tmin and tmax.
+template <bool WIDE = false>
 __device__ __forceinline__ float child_slab(const Ray &r, const float lo[3], const float hi[3], float limit) {
 #pragma clang fp contract(off)
 	float amin[3], amax[3];
@@ -142,7 +147,8 @@ __device__ __forceinline__ float child_slab(const Ray &r, const float lo[3], con
 	const float tmin = fmaxf(fmaxf(amin[0], amin[1]), amin[2]);
 	const float tmax = fminf(fminf(amax[0], amax[1]), amax[2]);
 	if (poison || r.patho) return -__builtin_inff();
-	const bool hit = (tmax >= 0.0f) && (tmin <= tmax) && (tmin <= limit);
+	const float slack = WIDE ? 0x1p-20f * (fabsf(tmin) + fabsf(tmax)) : 0.0f;
+	const bool hit = (tmax >= -slack) && (tmin <= tmax + slack) && (tmin <= limit);
 	return hit ? tmin : __builtin_nanf("");
 }
 
@@ -236,7 +242,7 @@ constexpr uint32_t kKeyNoCull = 0x00800000u;
 // the key kKeyNoCull | slot; the other slots of such a node are tested without the limit and
 // culled only when popped. `ncull_q` is the quantized node array, which holds the mask of
 // both formats (byte 15 of node idx).
-template <class Node, bool NC>
+template <class Node, bool NC, bool WIDE>
 __device__ __forceinline__ void test_children(const Node *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
                                               float limit, bool fast_ok, NodeHits &h);
 
@@ -254,11 +260,12 @@ __device__ __forceinline__ void no_cull_keys(uint32_t key[8], uint32_t ncull) {
 }
 
 // NC = false: the scene has no flagged triangle (every mask is 0) and the masks are not read.
-template <class Node, bool NC>
+// WIDE: widened reference-formula intervals (child_slab).
+template <class Node, bool NC, bool WIDE>
 struct ChildTest;
 
-template <bool NC>
-struct ChildTest<NodeQ, NC> {
+template <bool NC, bool WIDE>
+struct ChildTest<NodeQ, NC, WIDE> {
 static __device__ __forceinline__ void run(const NodeQ *nodes, const NodeQ * /*ncull_q*/, uint32_t idx, const Ray &r,
                                            float limit, bool fast_ok, NodeHits &h) {
 	const uint4 *src = reinterpret_cast<const uint4 *>(nodes + idx);
@@ -336,7 +343,7 @@ static __device__ __forceinline__ void run(const NodeQ *nodes, const NodeQ * /*n
 			lo[ax] = fmaf(static_cast<float>(bl), s[ax], p[ax]);
 			hi[ax] = fmaf(static_cast<float>(bh), s[ax], p[ax]);
 		}
-		const float tmin = child_slab(r, lo, hi, ncull != 0u ? __builtin_inff() : limit);
+		const float tmin = child_slab<WIDE>(r, lo, hi, ncull != 0u ? __builtin_inff() : limit);
 		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		h.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
@@ -345,8 +352,8 @@ static __device__ __forceinline__ void run(const NodeQ *nodes, const NodeQ * /*n
 }
 };
 
-template <bool NC>
-struct ChildTest<NodeF, NC> {
+template <bool NC, bool WIDE>
+struct ChildTest<NodeF, NC, WIDE> {
 static __device__ __forceinline__ void run(const NodeF *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
                                            float limit, bool /*fast_ok*/, NodeHits &hh) {
 	const NodeF *n = nodes + idx;
@@ -371,7 +378,7 @@ static __device__ __forceinline__ void run(const NodeF *nodes, const NodeQ *ncul
 		const uint32_t meta = ((k < 4 ? h.z : h.w) >> (8 * (k & 3))) & 0xFFu;
 		const float l[3] = {lo[0][k], lo[1][k], lo[2][k]};
 		const float u[3] = {hi[0][k], hi[1][k], hi[2][k]};
-		const float tmin = child_slab(r, l, u, ncull != 0u ? __builtin_inff() : limit);
+		const float tmin = child_slab<WIDE>(r, l, u, ncull != 0u ? __builtin_inff() : limit);
 		const bool hit = meta != kMetaEmpty && !__builtin_isnan(tmin);
 		hh.key[k] = hit ? sort_key(tmin, k) : kMiss;
 	}
@@ -380,10 +387,10 @@ static __device__ __forceinline__ void run(const NodeF *nodes, const NodeQ *ncul
 }
 };
 
-template <class Node, bool NC>
+template <class Node, bool NC, bool WIDE>
 __device__ __forceinline__ void test_children(const Node *nodes, const NodeQ *ncull_q, uint32_t idx, const Ray &r,
                                               float limit, bool fast_ok, NodeHits &h) {
-	ChildTest<Node, NC>::run(nodes, ncull_q, idx, r, limit, fast_ok, h);
+	ChildTest<Node, NC, WIDE>::run(nodes, ncull_q, idx, r, limit, fast_ok, h);
 }
 
 // Lane 0's value in every lane, as a wave-uniform (scalar) value; every lane of the wave must be
@@ -404,6 +411,7 @@ __device__ __forceinline__ void write_record(const KArgs &a, uint64_t ray_idx, c
 	write_any(a.mask, a.out + ray_idx * a.stride, h, r.o, r.d);
 }
 
+template <bool WT>
 __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ray &r) {
 	const uint32_t first = leaf_first(entry);
 	const uint32_t count = leaf_count(entry);
@@ -415,7 +423,8 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 		                    __uint_as_float(b.z), __uint_as_float(b.w), __uint_as_float(c.x)};
 		const uint32_t id = c.y;
 		float t, u, vv;
-		if (exact_candidate(v, r.o, r.d, r.inv, t, u, vv) && better(t, id, r.tb, r.idb)) {
+		const bool cand = WT ? wt_candidate(v, r.o, r.d, r.inv, t, u, vv) : exact_candidate(v, r.o, r.d, r.inv, t, u, vv);
+		if (cand && better(t, id, r.tb, r.idb)) {
 			r.tb = t;
 			r.ub = u;
 			r.vb = vv;
@@ -593,7 +602,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 					pend = kInvalid;
 					continue;
 				}
-				test_leaf(a.tris, leaf_entry(tri, 1u), r);
+				test_leaf<MODE == kWatertight>(a.tris, leaf_entry(tri, 1u), r);
 				if (COUNT) ++c_tris;
 				pend = left == 1u ? kInvalid : pend + 3u;  // next triangle, one fewer left
 			}
@@ -604,7 +613,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				continue;
 			}
 			NodeHits h;
-			test_children<Node, NC>(nodes, a.nodes_q, cur, r, cull_limit(r, a), a.cull && r.fast, h);
+			// the one-fma test's margins also cover the watertight mode's box corners
+			test_children<Node, NC, MODE == kWatertight>(nodes, a.nodes_q, cur, r, cull_limit(r, a),
+			                                             (MODE == kWatertight || a.cull) && r.fast, h);
 			if (COUNT) ++c_nodes;
 			cur = h.key[0] != kMiss ? h.entry(h.key[0]) : kInvalid;
 			// clusters searched in the seed phase are not searched again
@@ -655,6 +666,8 @@ template <class Node, int S, int B, int MODE>
 void launch_m(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
 	if constexpr (MODE == kBatch) {
 		launch_nc<Node, S, B, MODE, true>(a, grid, count, st);
+	} else if constexpr (MODE == kWatertight) {  // no distance culling: the masks are not read
+		launch_nc<Node, S, B, MODE, false>(a, grid, count, st);
 	} else {
 		if (a.any_ncull) launch_nc<Node, S, B, MODE, true>(a, grid, count, st);
 		else launch_nc<Node, S, B, MODE, false>(a, grid, count, st);
@@ -665,6 +678,7 @@ template <class Node, int S, int B>
 hipError_t launch(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
 	if (a.tiles) launch_m<Node, S, B, kBatch>(a, grid, count, st);
 	else if (a.seeds) launch_m<Node, S, B, kSeeded>(a, grid, count, st);
+	else if (a.leaf_test == 1) launch_m<Node, S, B, kWatertight>(a, grid, count, st);
 	else launch_m<Node, S, B, kPlain>(a, grid, count, st);
 	return hipGetLastError();
 }
@@ -750,7 +764,10 @@ hipError_t run_trace(const Bvh8 &b, const TraceParams &p, const float *rays, uin
 	a.refill = static_cast<uint32_t>(std::max(1, std::min(64, p.refill)));
 	a.margin_rel = p.margin_rel;
 	a.margin_abs = p.margin_abs;
-	a.cull = p.cull;
+	a.leaf_test = p.leaf_test;
+	// the watertight test's t is not covered by the culling guard (derived for Moller-Trumbore)
+	a.cull = p.leaf_test == 1 ? 0 : p.cull;
+	if (p.leaf_test == 1 && (seeds || batch)) return hipErrorInvalidValue;  // plain traversal only
 	a.seeds = seeds ? seeds->seeds : nullptr;
 	a.cl_entry = seeds ? seeds->entry : nullptr;
 	a.cl_box = seeds ? seeds->box : nullptr;

@@ -493,3 +493,61 @@ def test_builders_and_collapse(gpu_ctx, builder, collapse):
     finally:
         gpu_ctx.set_option("builder", 1)
         gpu_ctx.set_option("collapse", 1)
+
+
+# ------------------------------------------------------------- watertight leaf test (optional)
+
+@pytest.mark.parametrize("name", EDGE + ["workload0", "workload1_primary", "workload1_bounce", "workload2"])
+def test_watertight_parity(gpu_ctx, name):
+    """PRT_OPT_LEAF_TEST = 1 (Woop-Benthin-Wald): records equal the oracle's watertight mode
+    (oracle.WATERTIGHT, lowest id at equal t) bit for bit, with quantized and fp32 nodes."""
+    g = load_golden(name if name.startswith("workload") else "edge_" + name)
+    tris = scenes.scene(int(name[8])) if name.startswith("workload") else g["tris"]
+    rays = g["rays"]
+    want = oracle.trace(tris, rays, oracle.LOWEST_ID | oracle.WATERTIGHT)
+    gpu_ctx.set_tris(tris)
+    gpu_ctx.set_option("leaf_test", 1)
+    try:
+        for fmt in (1, 0):
+            gpu_ctx.set_option("node_format", fmt)
+            got = gpu_ctx.nearest_hits(rays, FULL)
+            assert_same(got, want, what=f"{name} watertight fmt={fmt}")
+    finally:
+        gpu_ctx.set_option("leaf_test", 0)
+        gpu_ctx.set_option("node_format", 1)
+
+
+def test_watertight_closed_mesh(gpu_ctx):
+    """Rays from the centre of the closed icosphere through 16,384 of its vertices (shared by
+    five or six triangles): every one hits with the watertight test, equal to the oracle's brute
+    force; the default Moller-Trumbore test lets some through (the reference's behaviour)."""
+    tris = scenes.scene(1)
+    verts = np.unique(tris.reshape(-1, 3), axis=0)[:16384]
+    rays = np.concatenate([np.zeros_like(verts), verts], 1).astype(np.float32)
+    gpu_ctx.set_tris(tris)
+    gpu_ctx.set_option("leaf_test", 1)
+    try:
+        got = gpu_ctx.nearest_hits(rays, FULL)
+    finally:
+        gpu_ctx.set_option("leaf_test", 0)
+    assert (got["t"] < np.inf).all(), f"{int((got['t'] == np.inf).sum())} rays leaked"
+    assert_same(got, oracle.brute(tris, rays, mode=oracle.LOWEST_ID | oracle.WATERTIGHT), what="closed mesh")
+    mt = gpu_ctx.nearest_hits(rays, FULL)
+    assert_same(mt, oracle.trace(tris, rays, oracle.LOWEST_ID), what="closed mesh, Moller-Trumbore")
+    print(f"Moller-Trumbore: {int((mt['t'] == np.inf).sum())} of {len(rays)} vertex rays pass through", flush=True)
+
+
+def test_watertight_refused_where_unsupported(gpu_ctx):
+    """The learned traversal keeps the reference's leaf test: it refuses the watertight option."""
+    gpu_ctx.set_tris(scenes.scene(0))
+    gpu_ctx.set_option("leaf_test", 1)
+    try:
+        with pytest.raises(prt.PrtError):
+            rays = scenes.rays(0, count=64)
+            import torch
+            d = torch.from_numpy(rays).cuda()
+            out = torch.empty(64 * 16, dtype=torch.uint8, device="cuda")
+            prt.learned_setup(gpu_ctx, 1)
+            prt.nearest_hits_learned_device(gpu_ctx, d.data_ptr(), 64, prt.T | prt.PRIMITIVE_ID, out.data_ptr())
+    finally:
+        gpu_ctx.set_option("leaf_test", 0)

@@ -221,3 +221,156 @@ def test_sliver_fixtures_are_adversarial(name):
             if j != win and hit and t_win < t < entry:
                 between.append(j)
         assert between, f"ray {i}: no competitor between {t_win} and {entry}"
+
+
+# ---------------------------------------------------------------- watertight leaf test (orc_wt)
+
+def test_wt_known_answer():
+    """The validation example's known answer (examples/validation/main.cpp:86-99: t = 1,
+    u = 0.3, v = 0.5) and its miss ray, with the watertight test; u, v are the weights of
+    vertices 1 and 2 as in Moller-Trumbore."""
+    tri = [-1, -1, 0, 1, -1, 0, 0, 1, 0]
+    hit, t, u, v = oracle.wt(tri, [0.1, 0, -1, 0, 0, 1])
+    assert hit and t == 1.0 and abs(u - 0.3) < 1e-6 and abs(v - 0.5) < 1e-6
+    assert not oracle.wt(tri, [-2, 0, -1, 0, 0, 1])[0]
+    # both windings, t without a range check (behind the origin), like the reference
+    hit, t, _, _ = oracle.wt(tri[3:6] + tri[0:3] + tri[6:9], [0.1, 0, 1, 0, 0, 1])
+    assert hit and t == -1.0
+    # every axis as the dominant one, both signs: rays aimed at an interior point give the FP64
+    # re-intersection's t, u, v (the weights of vertices 1 and 2)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        tri = rng.uniform(-1, 1, 9).astype(np.float32)
+        w = rng.dirichlet([1, 1, 1])
+        p = w[0] * tri[0:3] + w[1] * tri[3:6] + w[2] * tri[6:9]
+        d = rng.normal(size=3)
+        ray = np.concatenate([p - 3.0 * d, d]).astype(np.float32)
+        t64, u64, v64 = (x[0] for x in oracle.fp64(tri[None], ray[None], np.zeros(1, np.uint32)))
+        e1, e2 = tri[3:6].astype(np.float64) - tri[0:3], tri[6:9].astype(np.float64) - tri[0:3]
+        nrm = np.cross(e1, e2)
+        cos = abs(nrm @ d) / (np.linalg.norm(nrm) * np.linalg.norm(d))
+        if min(u64, v64, 1 - u64 - v64) > 1e-2 and cos > 0.3 and np.linalg.norm(nrm) > 0.05:
+            hit, t, u, v = oracle.wt(tri, ray)
+            assert hit and abs(t - t64) < 1e-5 * abs(t64) and abs(u - u64) < 1e-4 and abs(v - v64) < 1e-4
+
+
+def _sheared(tri, ray):
+    """The watertight test's sheared 2-D vertex coordinates, float32 (for the exact-sign check)."""
+    f = np.float32
+    o, d = ray[:3].astype(f), ray[3:].astype(f)
+    kz = int(np.argmax(np.abs(d))) if np.abs(d).max() > 0 else 0
+    kz = 0 if abs(d[1]) <= abs(d[0]) and abs(d[2]) <= abs(d[0]) else (1 if abs(d[2]) <= abs(d[1]) else 2)
+    kx, ky = (kz + 1) % 3, (kz + 2) % 3
+    if d[kz] < 0:
+        kx, ky = ky, kx
+    sx, sy = d[kx] / d[kz], d[ky] / d[kz]
+    out = []
+    for i in range(3):
+        a = tri[3 * i:3 * i + 3].astype(f) - o
+        out.append((a[kx] - sx * a[kz], a[ky] - sy * a[kz]))
+    return out
+
+
+def test_wt_edge_signs_are_exact():
+    """Accept/reject of rays aimed at a shared edge equals the sign test evaluated in exact
+    rational arithmetic on the same float32 sheared coordinates (a plain float32 evaluation of
+    the edge functions, without the fma-exact fallback, gives zeros where the exact value is
+    not zero and fails this); and a ray through a shared edge is accepted by at least one of the
+    two triangles."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(11)
+    checked = zero_rounded = 0
+    for _ in range(3000):
+        a, b = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        d = rng.normal(size=3)
+        # c1 and c2 on opposite sides of the edge as seen along the ray: together the two
+        # triangles cover a neighbourhood of every interior point of the edge
+        c1 = rng.uniform(-1, 1, 3)
+        side = np.cross(b - a, c1 - a) @ d
+        while True:
+            c2 = rng.uniform(-1, 1, 3)
+            if (np.cross(b - a, c2 - a) @ d) * side < 0:
+                break
+        t1 = np.concatenate([a, b, c1]).astype(np.float32)
+        t2 = np.concatenate([b, a, c2]).astype(np.float32)  # shares edge (a, b), opposite order
+        s = rng.uniform(0.05, 0.95)
+        p = t1[0:3].astype(np.float64) * (1 - s) + t1[3:6].astype(np.float64) * s
+        ray = np.concatenate([p - 2.0 * d, d]).astype(np.float32)
+        hits = []
+        for tri in (t1, t2):
+            (ax, ay), (bx, by), (cx, cy) = _sheared(tri, ray)
+            U = Fr(float(cx)) * Fr(float(by)) - Fr(float(cy)) * Fr(float(bx))
+            V = Fr(float(ax)) * Fr(float(cy)) - Fr(float(ay)) * Fr(float(cx))
+            W = Fr(float(bx)) * Fr(float(ay)) - Fr(float(by)) * Fr(float(ax))
+            if np.float32(cx) * np.float32(by) - np.float32(cy) * np.float32(bx) == 0 and U != 0:
+                zero_rounded += 1
+            want = not ((U < 0 or V < 0 or W < 0) and (U > 0 or V > 0 or W > 0)) and (U + V + W) != 0
+            got = oracle.wt(tri, ray)[0]
+            # det == 0 is tested on the rounded sum; only same-sign (accepted) cases matter here
+            if U + V + W != 0:
+                assert got == want, (tri, ray, U, V, W)
+                checked += 1
+            hits.append(got)
+        assert hits[0] or hits[1], "a ray through the shared edge went through both triangles"
+    assert checked > 5000
+
+
+def test_wt_closed_mesh_has_no_leaks():
+    """Rays from the centre of the closed icosphere (81,920 triangles) through every vertex
+    shared by five or six triangles: each hits the mesh with the watertight test (brute force
+    over all triangles); the reference's Moller-Trumbore lets some of them through."""
+    tris = scenes.scene(1)
+    verts = np.unique(tris.reshape(-1, 3), axis=0)[:4096]
+    rays = np.concatenate([np.zeros_like(verts), verts], 1).astype(np.float32)
+    wt = oracle.brute(tris, rays, mode=oracle.LOWEST_ID | oracle.WATERTIGHT)
+    assert (wt["t"] < np.inf).all(), f"{int((wt['t'] == np.inf).sum())} rays leaked"
+    assert np.allclose(wt["t"], 1.0, atol=1e-5)  # they reach the mesh at the vertex
+    mt = oracle.brute(tris, rays)
+    print(f"Moller-Trumbore misses {int((mt['t'] == np.inf).sum())} of {len(rays)} vertex rays")
+
+
+def test_wt_agrees_with_mt_and_fp64_away_from_edges():
+    """On random (ray, triangle) pairs whose hit lies clearly inside the triangle, the watertight
+    test accepts exactly when Moller-Trumbore does and both agree with the FP64 re-intersection
+    (t to 2e-5 relative, u and v to 5e-5), for rays at least 11.5 degrees off the plane."""
+    rng = np.random.default_rng(5)
+    n = 4000
+    tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+    # aim at barycentric points in [-0.2, 1.2]: inside and outside the triangle
+    w = rng.uniform(-0.2, 1.2, (n, 2))
+    tv = tris.astype(np.float64)
+    aim = tv[:, 0:3] + w[:, :1] * (tv[:, 3:6] - tv[:, 0:3]) + w[:, 1:] * (tv[:, 6:9] - tv[:, 0:3])
+    d = rng.normal(size=(n, 3))
+    rays = np.concatenate([aim - rng.uniform(0.5, 4, (n, 1)) * d, d], 1).astype(np.float32)
+    t64, u64, v64 = oracle.fp64(tris, rays, np.arange(n, dtype=np.uint32))
+    e1, e2 = tv[:, 3:6] - tv[:, 0:3], tv[:, 6:9] - tv[:, 0:3]
+    nrm = np.cross(e1, e2)
+    cos = np.abs((nrm * d).sum(1)) / (np.linalg.norm(nrm, axis=1) * np.linalg.norm(d, axis=1))
+    both = 0
+    for i in range(n):
+        if cos[i] < 0.2:  # grazing: both tests' rounding is large, acceptance near edges differs
+            continue
+        inside = min(u64[i], v64[i], 1 - u64[i] - v64[i]) > 1e-4
+        outside = min(u64[i], v64[i], 1 - u64[i] - v64[i]) < -1e-4
+        hw, tw, uw, vw = oracle.wt(tris[i], rays[i])
+        hm, tm, um, vm = oracle.mt(tris[i], rays[i])
+        if inside:
+            assert hw and hm
+            assert abs(tw - t64[i]) <= 2e-5 * max(1.0, abs(t64[i])) and abs(uw - u64[i]) < 5e-5 and abs(vw - v64[i]) < 5e-5
+            both += 1
+        elif outside:
+            assert not hw and not hm
+    assert both > 300
+
+
+def test_wt_traversal_equals_brute_force():
+    """The BVH2 traversal with the watertight leaf test equals brute force (lowest id) on random
+    soups, rays through vertices included."""
+    rng = np.random.default_rng(9)
+    for n in (7, 60, 500):
+        tris = (rng.uniform(-1, 1, (n, 1, 3)) + rng.uniform(-0.3, 0.3, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+        o = rng.uniform(-1.5, 1.5, (600, 3))
+        d = np.concatenate([rng.normal(size=(400, 3)), tris[np.arange(200) % n, 0:3] - o[400:]], 0)
+        rays = np.concatenate([o, d], 1).astype(np.float32)
+        mode = oracle.LOWEST_ID | oracle.WATERTIGHT
+        assert_same(oracle.trace(tris, rays, mode), oracle.brute(tris, rays, mode=mode), what=f"wt n={n}")
