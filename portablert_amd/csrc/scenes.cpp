@@ -136,7 +136,10 @@ double value_noise(const double p[3], uint64_t seed) {
 	return acc;
 }
 
-void gen_icosphere(uint64_t seed, float *out) {
+// Icosphere subdivided `levels` times (20 * 4^levels triangles), vertices at
+// centre + radius * (1 + amp * value_noise(freq * unit position)) along the unit direction.
+void gen_icosphere(uint64_t seed, int levels, const double centre[3], double radius, double amp, double freq,
+                   float *out) {
 	const double t = (1.0 + std::sqrt(5.0)) / 2.0;
 	std::vector<std::array<double, 3>> v = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0},
 	                                        {0, -1, t}, {0, 1, t}, {0, -1, -t}, {0, 1, -t},
@@ -149,7 +152,7 @@ void gen_icosphere(uint64_t seed, float *out) {
 	    {0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
 	    {11, 10, 2}, {10, 7, 6}, {7, 1, 8},   {3, 9, 4},  {3, 4, 2},   {3, 2, 6}, {3, 6, 8},
 	    {3, 8, 9},  {4, 9, 5},  {2, 4, 11},  {6, 2, 10}, {8, 6, 7},   {9, 8, 1}};
-	for (int level = 0; level < 6; ++level) {
+	for (int level = 0; level < levels; ++level) {
 		std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
 		auto midpoint = [&](uint32_t a, uint32_t b) {
 			const auto key = std::make_pair(a < b ? a : b, a < b ? b : a);
@@ -178,12 +181,29 @@ void gen_icosphere(uint64_t seed, float *out) {
 	}
 	std::vector<std::array<double, 3>> disp(v.size());
 	for (size_t i = 0; i < v.size(); ++i) {
-		const double q[3] = {4.0 * v[i][0], 4.0 * v[i][1], 4.0 * v[i][2]};
-		const double r = 1.0 + 0.08 * value_noise(q, seed);
-		for (int k = 0; k < 3; ++k) disp[i][k] = r * v[i][k];
+		const double q[3] = {freq * v[i][0], freq * v[i][1], freq * v[i][2]};
+		const double r = radius * (1.0 + amp * value_noise(q, seed));
+		for (int k = 0; k < 3; ++k) disp[i][k] = centre[k] + r * v[i][k];
 	}
 	for (size_t i = 0; i < f.size(); ++i)
 		put_tri(out + 9 * i, disp[f[i][0]].data(), disp[f[i][1]].data(), disp[f[i][2]].data());
+}
+
+// The displaced unit icosphere of workload 1: 6 subdivisions, 81,920 triangles.
+void gen_sphere_w1(float *out) {
+	const double c[3] = {0.0, 0.0, 0.0};
+	gen_icosphere(1, 6, c, 1.0, 0.08, 4.0, out);
+}
+
+// A closed stand-in for the reference's validation mesh (examples/common/bunny.obj, which has no
+// licence file and stays out of this repository), used with its cameras (workloads 5 and 6):
+// an icosphere subdivided 4 times (5,120 triangles; the bunny has 4,968) centred in the bunny's
+// bounding box ([-0.094, 0.061] x [0.033, 0.187] x [-0.062, 0.059]), radius 0.07, displaced by
+// 0.25 * value noise (seed 5, frequency 3).
+constexpr uint64_t kBlobTris = 5120;
+void gen_blob(float *out) {
+	const double c[3] = {-0.017, 0.11, -0.0015};
+	gen_icosphere(5, 4, c, 0.07, 0.25, 3.0, out);
 }
 
 // ---------------------------------------------------------------- room (workloads 2 and 4)
@@ -343,6 +363,28 @@ void bunny_ray(uint64_t idx, float *out) {
 	}
 }
 
+// The camera of the bunny benchmark (examples/bunny/main.cpp:84-122): 4096 x 4096 pixels, eye
+// at (0, -0.02, -0.3), sensor 0.05 wide at distance 0.05, directions of unit length (divided by
+// their own length, in float as written there), rows from the top.
+void bunny_bench_ray(uint64_t idx, float *out) {
+	const int width = 4096, height = 4096;
+	const int row = static_cast<int>(idx / width);
+	const int x = static_cast<int>(idx % width);
+	const int y = height - 1 - row;
+	const float camera_dist = 0.3f, sensor_size = 0.05f, sensor_dist = 0.05f;
+	const float sx = static_cast<float>(sensor_size * (static_cast<float>(x) / width - 0.5));
+	const float sy = static_cast<float>(sensor_size * (static_cast<float>(y) / height - 0.5));
+	const float cam[3] = {0.0f, -0.02f, -camera_dist};
+	const float sensor[3] = {sx, sy, -camera_dist + sensor_dist};
+	float d[3] = {sensor[0] - cam[0], sensor[1] - cam[1], sensor[2] - cam[2]};
+	const float len = static_cast<float>(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+	for (float &c : d) c /= len;
+	for (int k = 0; k < 3; ++k) {
+		out[k] = cam[k];
+		out[3 + k] = d[k];
+	}
+}
+
 // Cosine-weighted hemisphere direction about unit normal n (branchless orthonormal basis of
 // Duff et al. 2017).
 void cosine_dir(Rng &r, const double n[3], double d[3]) {
@@ -374,6 +416,8 @@ uint64_t prt_scene_tris(int workload, int scene) {
 	case 2:
 	case 4: return 262144;
 	case 3: return (scene >= 0 && scene < kBatchScenes) ? batch_tris(static_cast<uint64_t>(scene)) : 0;
+	case 5:
+	case 6: return kBlobTris;
 	default: return 0;
 	}
 }
@@ -382,7 +426,9 @@ uint64_t prt_scene_tris(int workload, int scene) {
 int prt_scene_generate(int workload, int scene, float *tris) {
 	switch (workload) {
 	case 0: gen_soup(0, 16384, tris); return 0;
-	case 1: gen_icosphere(1, tris); return 0;
+	case 1: gen_sphere_w1(tris); return 0;
+	case 5:
+	case 6: gen_blob(tris); return 0;
 	case 2:
 	case 4: gen_room(2, 262144, tris); return 0;
 	case 3:
@@ -403,6 +449,7 @@ uint64_t prt_rays_count(int workload, int scene, int stream) {
 	case 3: return (scene >= 0 && scene < kBatchScenes) ? batch_rays(static_cast<uint64_t>(scene)) : 0;
 	case 4: return stream == 1 ? 16777216ull : 8388608ull;
 	case 5: return 1024ull * 1024ull;
+	case 6: return 4096ull * 4096ull;
 	default: return 0;
 	}
 }
@@ -422,6 +469,7 @@ int prt_rays_generate(int workload, int scene, int stream, uint64_t first, uint6
 		case 3: soup_interior_ray(mix64(3 * 7919ull + static_cast<uint64_t>(scene)), 1, idx, o); break;
 		case 4: room_ray(4, static_cast<uint64_t>(stream), idx, o); break;
 		case 5: bunny_ray(idx, o); break;
+		case 6: bunny_bench_ray(idx, o); break;
 		default: return 1;
 		}
 	}

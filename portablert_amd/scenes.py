@@ -2,7 +2,9 @@
 
 Thin ctypes view of ``lib/libprt_scenes.so`` (csrc/scenes.cpp). Workload numbers:
 0 soup + pinhole, 1 displaced icosphere (+ bounce rays), 2 room with incoherent rays,
-3 batch of 64 soups, 4 room with training / inference ray streams, 5 bunny camera.
+3 batch of 64 soups, 4 room with training / inference ray streams, 5 the validation example's
+1024 x 1024 bunny camera and 6 the bunny benchmark's 4096 x 4096 camera, both on a closed
+5,120-triangle stand-in for the bunny mesh (the mesh itself has no licence file).
 """
 import ctypes
 import hashlib
@@ -13,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libprt_scenes.so")
 
-SOUP, SPHERE, ROOM, BATCH, LEARNED, BUNNY = 0, 1, 2, 3, 4, 5
+SOUP, SPHERE, ROOM, BATCH, LEARNED, BUNNY, BUNNY_BENCH = 0, 1, 2, 3, 4, 5, 6
 
 _lib = None
 

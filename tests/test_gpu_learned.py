@@ -134,3 +134,40 @@ def test_refit_requires_learned_setup_again(gpu_ctx, room):
     assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
     hit = want["t"] < np.inf
     assert np.array_equal(got["primitive_id"][hit], want["primitive_id"][hit])
+
+
+def test_learned_config_size(gpu_ctx, room):
+    """The learned configuration at its stated size: trained on all 16,777,216 training rays
+    (one epoch), then the learned-seeded traversal equals the plain traversal byte for byte on
+    all 8,388,608 fresh rays, and both equal the oracle on every ray."""
+    import torch
+    prt.learned_setup(gpu_ctx, 7)
+    train = scenes.rays(4, stream=1)
+    assert len(train) == 16777216
+    d_train = torch.from_numpy(train).cuda()
+    torch.cuda.synchronize()
+    del train
+    print("training on 16,777,216 rays", flush=True)
+    prt.learned_train(gpu_ctx, d_train.data_ptr(), len(d_train), epochs=1, batch=8192, lr=1e-3)
+    del d_train
+    torch.cuda.empty_cache()
+    test = scenes.rays(4, stream=2)
+    assert len(test) == 8388608
+    d_test = torch.from_numpy(test).cuda()
+    top1, top4 = prt.learned_eval(gpu_ctx, d_test.data_ptr(), len(test))
+    print(f"trained: top-1 {top1:.3f} top-4 {top4:.3f}", flush=True)
+    assert top4 > 0.5
+    mask = prt.T | prt.PRIMITIVE_ID
+    size, _ = prt.layout(mask)
+    out_plain = torch.zeros(len(test) * size, dtype=torch.uint8, device="cuda")
+    out_learn = torch.zeros_like(out_plain)
+    torch.cuda.synchronize()
+    gpu_ctx.nearest_hits_device(d_test.data_ptr(), len(test), mask, out_plain.data_ptr())
+    prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), len(test), mask, out_learn.data_ptr())
+    gpu_ctx.synchronize()
+    assert torch.equal(out_plain, out_learn), "learned differs from plain on the 8,388,608 fresh rays"
+    got = out_learn.cpu().numpy().view(prt.hit_dtype(mask))
+    want = oracle.trace(room, test, oracle.LOWEST_ID)
+    assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+    hit = want["t"] < np.inf
+    assert np.array_equal(got["primitive_id"][hit], want["primitive_id"][hit])

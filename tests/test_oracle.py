@@ -374,3 +374,15 @@ def test_wt_traversal_equals_brute_force():
         rays = np.concatenate([o, d], 1).astype(np.float32)
         mode = oracle.LOWEST_ID | oracle.WATERTIGHT
         assert_same(oracle.trace(tris, rays, mode), oracle.brute(tris, rays, mode=mode), what=f"wt n={n}")
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="compiled reference not built (needs the reference sources)")
+def test_bunny_stand_in_equals_reference():
+    """The oracle on the validation camera (workload 5, non-unit directions, zero components)
+    and the closed stand-in mesh equals the compiled reference backend on every ray (every
+    field; primitive_id where the reference has no equal-t tie)."""
+    tris, rays = scenes.scene(5), scenes.rays(5)
+    ref = oracle.ref_trace(tris, rays)
+    assert_same(oracle.trace(tris, rays, oracle.DFS), ref, what="bunny stand-in, depth-first")
+    assert_same(oracle.trace(tris, rays, oracle.LOWEST_ID), ref, ids=False, what="bunny stand-in, lowest id")
+    assert 0.05 < float((ref["t"] < np.inf).mean()) < 0.2
