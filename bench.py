@@ -157,11 +157,26 @@ def learned_leg(ctx, tris, steps, warmup, epochs, train_rays=None):
         return (time.perf_counter() - t) / steps
 
     plain_s = timed(lambda: ctx.nearest_hits_device(d_test.data_ptr(), n, mask, out_p.data_ptr(), sp))
+    # the number of predicted clusters searched first: the fastest seeded traversal of 1..4
+    seeded_by_n = {}
+    ctx.set_option("time_kernel", 1)
+    for k in (1, 2, 4):
+        ctx.set_option("seeds", k)
+        ms = []
+        for _ in range(3):
+            prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp)
+            ms.append(ctx.stat("seeded_ms"))
+        seeded_by_n[k] = float(np.median(ms))
+    ctx.set_option("time_kernel", 0)
+    best_k = min(seeded_by_n, key=seeded_by_n.get)
+    ctx.set_option("seeds", best_k)
     learn_s = timed(lambda: prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp))
     identical = bool(torch.equal(out_p, out_l))
     ctx.set_option("time_kernel", 1)
     prt.nearest_hits_learned_device(ctx, d_test.data_ptr(), n, mask, out_l.data_ptr(), sp)
     mlp_ms, seeded_ms = ctx.stat("mlp_ms"), ctx.stat("seeded_ms")
+    ctx.nearest_hits_device(d_test.data_ptr(), n, mask, out_p.data_ptr(), sp)
+    plain_kernel_ms = ctx.stat("trace_ms")
     ctx.set_option("time_kernel", 0)
     ctx.set_option("count_stats", 1)
     ctx.nearest_hits_device(d_test.data_ptr(), n, mask, out_p.data_ptr(), sp)
@@ -172,6 +187,7 @@ def learned_leg(ctx, tris, steps, warmup, epochs, train_rays=None):
     nodes_seeded = ctx.stat("node_visits") / n
     ctx.set_option("count_stats", 0)
     flops = 2 * (192 * 256 + 2 * 256 * 256 + 256 * 4097) * n
+    peak_ms = flops / 2.5e15 * 1e3  # the network at the dense BF16 MFMA peak
     res = {
         "rays": f"room scene; {len(d_train):,} training rays, 8,388,608 fresh rays (uniform origins, uniform directions)",
         "epochs": epochs,
@@ -184,6 +200,14 @@ def learned_leg(ctx, tris, steps, warmup, epochs, train_rays=None):
         "mlp_ms": round(mlp_ms, 3),
         "seeded_traversal_ms": round(seeded_ms, 3),
         "mlp_tflops": round(flops / (mlp_ms * 1e-3) / 1e12, 1) if mlp_ms > 0 else None,
+        "mlp_frac_of_bf16_peak": round(flops / (mlp_ms * 1e-3) / 2.5e15, 4) if mlp_ms > 0 else None,
+        "seeds_searched_first": best_k,
+        "seeded_traversal_ms_by_seeds": {str(k): round(v, 3) for k, v in seeded_by_n.items()},
+        "plain_traversal_kernel_ms": round(plain_kernel_ms, 3),
+        # learned wins only when the plain traversal takes longer than the learned call: with the
+        # network as measured, and with the network at the dense BF16 peak (2.5 PFLOP/s)
+        "break_even_plain_ms": round(mlp_ms + seeded_ms, 3),
+        "break_even_plain_ms_at_mfma_peak": round(peak_ms + seeded_ms, 3),
         "seed_top1": round(top1, 4),
         "seed_top4": round(top4, 4),
         "seed_top4_untrained": round(untrained[1], 5),

@@ -55,9 +55,11 @@ enum {
 	PRT_OPT_REFILL = 14,      /* a wave loads new rays once at least v of its lanes are idle, 1..64 (default 8) */
 	PRT_OPT_COLLAPSE = 15,    /* binary tree to BVH8: 0 greedy (open the largest child until full), 1
 	                             least surface-area-heuristic cost (default); at set_tris */
-	PRT_OPT_LEAF_TEST = 16    /* 0 Moller-Trumbore, the reference's test (default); 1 watertight
+	PRT_OPT_LEAF_TEST = 16,   /* 0 Moller-Trumbore, the reference's test (default); 1 watertight
 	                             (Woop, Benthin, Wald 2013; plain traversal only, no distance
 	                             culling) */
+	PRT_OPT_SEEDS = 17        /* learned traversal: the best 1..4 predicted clusters searched
+	                             first (default 4) */
 };
 
 /* Statistics (prt_mi355x_get_stat). */

@@ -23,7 +23,7 @@ OK, ERR_ARG, ERR_NODEV, ERR_HIP, ERR_OOM, ERR_STATE, ERR_LAYOUT, ERR_INPUT = ran
 
 OPT = dict(node_format=1, cull=2, leaf_size=3, block=4, waves_per_cu=5, lds_stack=6, chunk=7, leaf_phase=8,
            count_stats=9, time_kernel=10, alloc_limit=11, margin_rel_log2=12, builder=13, refill=14,
-           collapse=15, leaf_test=16)
+           collapse=15, leaf_test=16, seeds=17)
 STAT = dict(build_ms=1, trace_ms=2, nodes=3, depth=4, node_visits=5, tri_tests=6, leaf_visits=7,
             node_bytes=8, api_ms=9, n_tris=10, node_phases=11, tri_phases=12, culled_pops=13, refit_ms=21, ncull_tris=22)
 

@@ -525,6 +525,10 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		if (v != 0 && v != 1) return bad();
 		c->tp.leaf_test = static_cast<int>(v);
 		break;
+	case PRT_OPT_SEEDS:
+		if (v < 1 || v > 4) return bad();
+		c->tp.n_seeds = static_cast<int>(v);
+		break;
 	case PRT_OPT_BUILDER:
 		if (v != 0 && v != 1) return bad();
 		c->bp.builder = static_cast<int>(v);
@@ -557,6 +561,7 @@ int prt_mi355x_get_option(const prt_ctx *c, int opt, int64_t *v) {
 	case PRT_OPT_BUILDER: *v = c->bp.builder; break;
 	case PRT_OPT_COLLAPSE: *v = c->bp.collapse; break;
 	case PRT_OPT_LEAF_TEST: *v = c->tp.leaf_test; break;
+	case PRT_OPT_SEEDS: *v = c->tp.n_seeds; break;
 	default: return PRT_ERR_ARG;
 	}
 	return PRT_OK;

@@ -55,6 +55,7 @@ struct KArgs {
 	const uint32_t *cl_entry;
 	const float *cl_box;
 	uint32_t n_clusters;
+	int n_seeds;            // predicted clusters searched first (1..4, the best ones)
 	// several scenes (null when off): ray tiles {first lo, first hi, count, scene} claimed one
 	// per atomic, and per scene {margin_abs, fast_scene}; BVH8 node s is the root of scene s
 	const uint4 *tiles;
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 #pragma unroll
 					for (int q = 0; q < 4; ++q) {
 						sd[q] = kInvalid;
-						if (sc[q] < a.n_clusters) {
+						if (sc[q] < a.n_clusters && q >= 4 - a.n_seeds) {  // the best n_seeds only
 							const float *bx = a.cl_box + 6ull * sc[q];
 							const float lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
 							const float tmin = child_slab(r, lo, hi, __builtin_inff());
@@ -554,10 +555,22 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 		}
 		if (!active) continue;
 
-		// ---- fetch work: pop one entry; park a leaf if the parking slot is free
+		// clusters searched in the seed phase are not searched again: their entries are dropped
+		// when the root traversal takes them (from the stack or as a node's nearest child)
+		auto searched = [&](uint32_t e) {
+			if (!SEEDED || !root_phase) return false;
+			bool s = false;
+#pragma unroll
+			for (int q = 0; q < 4; ++q) s |= ((seen >> q) & 1u) && e == sd[q];
+			return s;
+		};
+
+		// ---- fetch work: pop one entry (a culled one leaves the lane without work for this
+		// iteration: no divergent loop over consecutive culled entries); park a leaf if the
+		// parking slot is free
 		if (cur == kInvalid && st.sp > 0) {
 			const uint32_t klim = key_limit(cull_limit(r, a));
-			do {
+			{
 				const uint2 e = st.pop();
 				if (e.y <= klim) {
 					cur = e.x;
@@ -566,11 +579,13 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 						if (cur == 0u) root_phase = true;
 #pragma unroll
 						for (int q = 0; q < 4; ++q) seen |= (cur == sd[q] && cur != 0u) ? (1u << q) : 0u;
+					} else if (searched(cur)) {
+						cur = kInvalid;
 					}
 				} else if (COUNT) {
 					++c_culled;
 				}
-			} while (cur == kInvalid && st.sp > 0);
+			}
 		}
 		if (cur != kInvalid && (cur & kLeafBit) && pend == kInvalid) {
 			pend = cur;
@@ -582,9 +597,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 			continue;
 		}
 
-		// ---- one phase per iteration for the whole wave: nodes, or one triangle per parked
-		// leaf. Lanes with a parked leaf keep traversing nodes meanwhile; the triangle phase
-		// runs when no lane has a node to visit or enough lanes have parked a leaf.
+		// ---- one phase per iteration for the whole wave: nodes, or the triangles of every
+		// parked leaf. Lanes with a parked leaf keep traversing nodes meanwhile; the triangle
+		// phase runs when no lane has a node to visit or enough lanes have parked a leaf.
 		const uint64_t want_node = __ballot(cur != kInvalid && !(cur & kLeafBit));
 		const uint64_t parked = __ballot(pend != kInvalid);
 		const uint64_t live = __ballot(true);
@@ -602,9 +617,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 					pend = kInvalid;
 					continue;
 				}
-				test_leaf<MODE == kWatertight>(a.tris, leaf_entry(tri, 1u), r);
-				if (COUNT) ++c_tris;
-				pend = left == 1u ? kInvalid : pend + 3u;  // next triangle, one fewer left
+				test_leaf<MODE == kWatertight>(a.tris, pend, r);  // the whole leaf (1..4 triangles)
+				if (COUNT) c_tris += left;
+				pend = kInvalid;
 			}
 		} else if (cur != kInvalid && !(cur & kLeafBit)) {
 			if (cur >= a.n_nodes || st.sp + 7u > a.max_entries) {  // corrupt node or stack bound
@@ -618,14 +633,6 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 			                                             (MODE == kWatertight || a.cull) && r.fast, h);
 			if (COUNT) ++c_nodes;
 			cur = h.key[0] != kMiss ? h.entry(h.key[0]) : kInvalid;
-			// clusters searched in the seed phase are not searched again
-			auto searched = [&](uint32_t e) {
-				if (!SEEDED || !root_phase) return false;
-				bool s = false;
-#pragma unroll
-				for (int q = 0; q < 4; ++q) s |= ((seen >> q) & 1u) && e == sd[q];
-				return s;
-			};
 			if (searched(cur)) cur = kInvalid;
 			// farthest first, so the nearest remaining child is on top
 #pragma unroll
@@ -633,7 +640,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				if (h.key[k] != kMiss) {
 					const uint32_t e = h.entry(h.key[k]);
 					if (COUNT) c_leaves += (e & kLeafBit) ? 1u : 0u;
-					if (!searched(e)) st.push(e, h.key[k]);
+					st.push(e, h.key[k]);
 				}
 			}
 		}
@@ -772,6 +779,7 @@ hipError_t run_trace(const Bvh8 &b, const TraceParams &p, const float *rays, uin
 	a.cl_entry = seeds ? seeds->entry : nullptr;
 	a.cl_box = seeds ? seeds->box : nullptr;
 	a.n_clusters = seeds ? seeds->count : 0u;
+	a.n_seeds = std::max(1, std::min(4, p.n_seeds));
 	a.tiles = batch ? batch->tiles : nullptr;
 	a.n_tiles = batch ? batch->n_tiles : 0u;
 	a.scene_info = batch ? batch->info : nullptr;

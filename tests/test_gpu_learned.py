@@ -111,6 +111,28 @@ def test_learned_traversal_identical_to_plain(gpu_ctx, room):
     assert (got["primitive_id"][idx][hit] == want["primitive_id"][hit]).all()
 
 
+def test_seed_count_keeps_records(gpu_ctx, room):
+    """Searching the best 1, 2, 3 or 4 predicted clusters first changes only the order of the
+    search: records equal the plain traversal's byte for byte."""
+    import torch
+    test = scenes.rays(4, stream=2, first=4096, count=262144)
+    d_test = torch.from_numpy(test).cuda()
+    mask = prt.mask_of(prt.ALL_TAGS)
+    size, _ = prt.layout(mask)
+    plain = torch.zeros(len(test) * size, dtype=torch.uint8, device="cuda")
+    learn = torch.zeros_like(plain)
+    torch.cuda.synchronize()
+    gpu_ctx.nearest_hits_device(d_test.data_ptr(), len(test), mask, plain.data_ptr())
+    try:
+        for k in (1, 2, 3, 4):
+            gpu_ctx.set_option("seeds", k)
+            prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), len(test), mask, learn.data_ptr())
+            gpu_ctx.synchronize()
+            assert torch.equal(plain, learn), f"learned with {k} seeds differs from plain"
+    finally:
+        gpu_ctx.set_option("seeds", 4)
+
+
 def test_refit_requires_learned_setup_again(gpu_ctx, room):
     """Refit changes every box, so the cluster boxes of the learned state are stale: learned
     calls report ERR_STATE until learned_setup runs again."""
