@@ -539,6 +539,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-watertight", action="store_true")
     ap.add_argument("--no-bunny", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the fp32-node leg")
+    ap.add_argument("--no-api", action="store_true", help="skip the host-buffer (PCIe) entry point leg")
     ap.add_argument("--format", type=int, default=1, help="0 fp32 nodes, 1 quantized nodes")
     ap.add_argument("--cull", type=int, default=1)
     ap.add_argument("--calib", action="store_true",
@@ -639,14 +641,17 @@ def main():
     neg_t = float(np.mean(hits["t"] < 0))
 
     # the host-buffer entry point: PCIe copies of rays in and records out included
-    api_ms = []
-    for _ in range(5):  # median of 5; the call above was the warm-up
-        ctx.nearest_hits(rays, mask)
-        api_ms.append(ctx.stat("api_ms"))
-    api_ms = float(np.median(api_ms))
+    api_ms = None
+    if not args.no_api:
+        api = []
+        for _ in range(5):  # median of 5; the call above was the warm-up
+            ctx.nearest_hits(rays, mask)
+            api.append(ctx.stat("api_ms"))
+        api_ms = float(np.median(api))
 
     # precision: fp32 BVH8 nodes against 8-bit quantized nodes, same records
-    precision = precision_leg(ctx, d_rays, n, mask, d_out, stream, max(5, args.steps // 2))
+    precision = None if args.no_precision else precision_leg(ctx, d_rays, n, mask, d_out, stream,
+                                                             max(5, args.steps // 2))
     refit = None if args.no_refit else refit_leg(ctx, tris, d_rays, n, mask, stream, max(3, args.steps // 4))
 
     achieved = BYTES_PER_RAY * n / (kernel_ms * 1e-3) / 1e9
@@ -702,8 +707,8 @@ def main():
         "oracle_agreement": agree,
         "fp64_error": fp64,
         "negative_t_rate": neg_t,
-        "api_boundary_ms": round(api_ms, 3),
-        "api_boundary_mrays_s": round(n / api_ms / 1e3, 2),
+        "api_boundary_ms": round(api_ms, 3) if api_ms else None,
+        "api_boundary_mrays_s": round(n / api_ms / 1e3, 2) if api_ms else None,
         "precision": precision,
     }
     if refit is not None:

@@ -108,7 +108,9 @@ int prt_mi355x_device_name(const prt_ctx *ctx, char *buf, size_t cap);
  * Backend::set_tris (backend.hpp:19, intersect_cpu.cpp:12). */
 int prt_mi355x_set_tris(prt_ctx *ctx, const float *tris9, uint64_t n);
 
-/* Same with the triangles already in device memory (read during the call only). */
+/* Same with the triangles already in device memory (read during the call only). The copy runs
+ * on the context's own stream, which does not wait for other streams: the caller must have
+ * completed its writes to d_tris9 (synchronise the stream that wrote them) before the call. */
 int prt_mi355x_set_tris_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
 
 /* Refit for dynamic geometry: the same n triangles (same order and ids) at new positions. Keeps
@@ -121,7 +123,8 @@ int prt_mi355x_set_tris_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
  * and notes a move version as a to-do (backend.hpp:18-19, intersect_cpu.cpp:12). */
 int prt_mi355x_refit(prt_ctx *ctx, const float *tris9, uint64_t n);
 
-/* Same with the triangles in device memory. */
+/* Same with the triangles in device memory (their writes completed before the call, as for
+ * prt_mi355x_set_tris_device). */
 int prt_mi355x_refit_device(prt_ctx *ctx, const float *d_tris9, uint64_t n);
 
 /* Nearest hit of n rays in host memory, records written to host memory `out` with the
