@@ -657,16 +657,17 @@ def main():
     achieved = BYTES_PER_RAY * n / (kernel_ms * 1e-3) / 1e9
     traffic, valu = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_room.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and args.format == 1 and args.cull:
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch")  # counters of the committed profile
     vf = os.path.join(ROOT, "profiles", "valu_room.json")
-    if os.path.exists(vf) and args.format == 1:
+    if os.path.exists(vf) and args.format == 1 and args.cull:
         with open(vf) as f:
             vj = json.load(f)
         # the bound that holds: VALU issue. Instructions per launch from the committed counters of
-        # this workload; the ceiling weights the probe's int32 and fp32 issue rates by the measured
-        # instruction mix (gfx950 issues 32-bit integer VALU work about 1.8x faster than fp32).
+        # this workload; the ceiling weights the measured int32 and fp32 issue rates of this GPU by
+        # the kernel's instruction mix (gfx950 issues 32-bit integer adds about 1.8x faster than
+        # fp32 work).
         # The counters leave 42% of the instructions unclassified: at the int32 rate they give
         # the upper bound used as `peak` (frac is then a lower bound), at the fp32 rate the lower
         # bound `peak_lower`
@@ -677,7 +678,8 @@ def main():
                 "peak_lower": vj["peak_for_mix_g_per_s"],
                 "int32_share": vj["int32_share"], "unclassified_share": vj["unclassified_share"],
                 "lane_activity": vj["lane_activity"],
-                "source": "profiles/valu_room.json (rocprofv3 counters, portablert_amd/lib/valu_rate probe)"}
+                "source": "instructions and mix: profiles/valu_room.json (rocprofv3 counters of this "
+                          "workload); issue rates: profiles/valu_issue_rate.md; achieved uses kernel_ms"}
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -697,7 +699,11 @@ def main():
                    "node_format": "quantized8" if args.format else "fp32", "cull": bool(args.cull),
                    "traversal": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "formula": "achieved = 40 B/ray x 8,388,608 rays / kernel_ms (stream events over the timed "
+                                "steps, the per-step counter reset included); frac = achieved / peak",
+                     "traffic_source": "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch from "
+                                       "profiles/traffic_room.json (committed counters, not this run)"},
         "valu_roofline": valu,
         "build_ms": round(build_ms, 3),
         "kernel_ms": round(kernel_ms, 4),
