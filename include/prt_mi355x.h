@@ -45,7 +45,7 @@ enum {
 	PRT_OPT_WAVES_PER_CU = 5, /* persistent waves per CU (default 32) */
 	PRT_OPT_LDS_STACK = 6,    /* LDS stack entries per lane: 8 (default) or 16 */
 	PRT_OPT_CHUNK = 7,        /* rays a wave reserves per atomic, 64..65536 (default 128) */
-	PRT_OPT_LEAF_PHASE = 8,   /* triangle phase when parked leaves >= live lanes * v / 64, 1..64 (default 24) */
+	PRT_OPT_LEAF_PHASE = 8,   /* triangle phase when parked leaves >= live lanes * v / 64, 1..64 (default 20) */
 	PRT_OPT_COUNT_STATS = 9,  /* 1: count node visits / triangle tests (slower) */
 	PRT_OPT_TIME_KERNEL = 10, /* 1: time the traversal kernel with HIP events (synchronises) */
 	PRT_OPT_ALLOC_LIMIT = 11, /* refuse device allocations above this many bytes (0 = none) */

@@ -128,7 +128,7 @@ struct TraceParams {
 	int waves_per_cu = 32;    // persistent waves per CU
 	int lds_stack = 8;        // LDS stack entries per thread (power of two)
 	int chunk = 128;          // rays a wave reserves per atomic (refilled lane by lane)
-	int leaf_phase = 24;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
+	int leaf_phase = 20;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
 	int refill = 8;           // load new rays once at least this many lanes of a wave are idle
 	int count_stats = 0;      // count node visits and triangle tests
 	int leaf_test = 0;        // 0 Moller-Trumbore (reference), 1 watertight (no distance culling)
