@@ -339,7 +339,15 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
 	constexpr int kOutTiles = (kMlpOut + kFCols - 1) / kFCols;  // the last one holds the miss logit
+	// bias of this thread's two columns, loaded one tile ahead (padding columns: -inf, never
+	// taken into a top-4)
+	auto bias_of = [&](int t, int j) {
+		const int col = t * kFCols + wc + j * 16 + (lane & 15);
+		return col < kMlpOut ? a.b[3][col] : -__builtin_inff();
+	};
+	float bias[2] = {bias_of(0, 0), bias_of(0, 1)};
 	for (int t = 0; t < kOutTiles; ++t) {
+		const float bnext[2] = {bias_of(t + 1, 0), bias_of(t + 1, 1)};
 		// tile t is in Wt; the next goes to the other buffer (X's, free once its rows are in
 		// registers) during this tile's MFMAs. The output tile goes through LDS over Wt.
 		uint16_t *Wt = (t & 1) ? X : W, *Wn = (t & 1) ? W : X;
@@ -368,8 +376,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 #pragma unroll
 		for (int j = 0; j < 2; ++j) {
 			const int col = wc + j * 16 + (lane & 15);
-			// padding columns get -inf: never taken into a top-4
-			const float bj = t * kFCols + col < kMlpOut ? a.b[3][t * kFCols + col] : -__builtin_inff();
+			const float bj = bias[j];
 #pragma unroll
 			for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -408,6 +415,8 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 			}
 		}
 		__syncthreads();  // the output tile is read: Wt is free for tile t + 2
+		bias[0] = bnext[0];
+		bias[1] = bnext[1];
 	}
 	// merge the four parts of each ray (value descending, index ascending)
 	top_merge(v, ix, 1);
