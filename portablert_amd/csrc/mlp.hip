@@ -169,9 +169,10 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 // One workgroup (4 waves) takes 64 rays through the whole network: encoding, the three hidden
 // layers and the 4097 output logits, keeping the activations in LDS (X, 64 x 256 bf16) and
 // streaming W^T tiles of 64 output neurons (W, prefetched into registers one tile ahead) from
-// L2. Each wave computes a 32 x 32 part of every 64 x 64 tile with v_mfma_f32_16x16x32_bf16
-// over K in steps of 32. For the output layer each wave first loads its 32 rows of the last
-// hidden layer into registers, so the 66 output tiles read only W from LDS. Every output tile
+// L2. In the hidden layers each wave computes a 32 x 32 part of every 64 x 64 tile with
+// v_mfma_f32_16x16x32_bf16 over K in steps of 32. For the output layer every wave loads all 64
+// rows of the last hidden layer into registers and computes 16 of each tile's 64 columns, so
+// the 65 output tiles read only W from LDS, each W fragment once per 64 rows. Every output tile
 // goes through LDS (over its W buffer; W is double-buffered in X's space once X's rows are in
 // registers) to four row-owner threads per ray (16 columns each) that keep a running top-4
 // (value descending, index ascending); no logit leaves the chip. The workgroup uses 66 KB of
@@ -319,13 +320,15 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	hidden_layer<kMlpIn>(X, W, a.w[0], a.b[0], tid, wr, wc, lane);
 	hidden_layer<kMlpHidden>(X, W, a.w[1], a.b[1], tid, wr, wc, lane);
 	hidden_layer<kMlpHidden>(X, W, a.w[2], a.b[2], tid, wr, wc, lane);
-	// output layer: this wave's 32 rows of the last hidden layer in registers
-	bf16x8 ar[2][kMlpHidden / kBK];
+	// output layer: all 64 rows of the last hidden layer in every wave's registers; each wave
+	// computes 16 of a tile's 64 columns, so each W fragment read from LDS serves 64 rows
+	const int oc = wave * 16;
+	bf16x8 ar[4][kMlpHidden / kBK];
 #pragma unroll
-	for (int i = 0; i < 2; ++i)
+	for (int i = 0; i < 4; ++i)
 #pragma unroll
 		for (int kk = 0; kk < kMlpHidden / kBK; ++kk)
-			ar[i][kk] = *reinterpret_cast<const bf16x8 *>(X + (wr + i * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+			ar[i][kk] = *reinterpret_cast<const bf16x8 *>(X + (i * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
 	{
 		u32x4 p0[kMlpHidden / 32];
 		tile_fetch<kMlpHidden>(a.w[3], 0, tid, p0);
@@ -341,46 +344,36 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	constexpr int kOutTiles = (kMlpOut + kFCols - 1) / kFCols;  // the last one holds the miss logit
 	// bias of this thread's two columns, loaded one tile ahead (padding columns: -inf, never
 	// taken into a top-4)
-	auto bias_of = [&](int t, int j) {
-		const int col = t * kFCols + wc + j * 16 + (lane & 15);
+	auto bias_of = [&](int t) {
+		const int col = t * kFCols + oc + (lane & 15);
 		return col < kMlpOut ? a.b[3][col] : -__builtin_inff();
 	};
-	float bias[2] = {bias_of(0, 0), bias_of(0, 1)};
+	float bias = bias_of(0);
 	for (int t = 0; t < kOutTiles; ++t) {
-		const float bnext[2] = {bias_of(t + 1, 0), bias_of(t + 1, 1)};
+		const float bnext = bias_of(t + 1);
 		// tile t is in Wt; the next goes to the other buffer (X's, free once its rows are in
 		// registers) during this tile's MFMAs. The output tile goes through LDS over Wt.
 		uint16_t *Wt = (t & 1) ? X : W, *Wn = (t & 1) ? W : X;
 		float *Ct = reinterpret_cast<float *>(Wt);
 		u32x4 pre[kMlpHidden / 32];
 		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * kFCols, tid, pre);
-		f32x4 acc[2][2];
+		f32x4 acc[4];
 #pragma unroll
-		for (int i = 0; i < 2; ++i)
-#pragma unroll
-			for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+		for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
 		for (int kk = 0; kk < kMlpHidden / kBK; ++kk) {
-			bf16x8 bfr[2];
+			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(Wt + (oc + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
 #pragma unroll
-			for (int s = 0; s < 2; ++s)
-				bfr[s] = *reinterpret_cast<const bf16x8 *>(Wt + (wc + s * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
-#pragma unroll
-			for (int i = 0; i < 2; ++i)
-#pragma unroll
-				for (int j = 0; j < 2; ++j)
-					acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr[j], acc[i][j], 0, 0, 0);
+			for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr, acc[i], 0, 0, 0);
 		}
 		tile_store<kMlpHidden>(Wn, tid, pre);
 		__syncthreads();  // every wave is done reading Wt; the next tile is stored
+		{
+			const int col = oc + (lane & 15);
 #pragma unroll
-		for (int j = 0; j < 2; ++j) {
-			const int col = wc + j * 16 + (lane & 15);
-			const float bj = bias[j];
+			for (int i = 0; i < 4; ++i)
 #pragma unroll
-			for (int i = 0; i < 2; ++i)
-#pragma unroll
-				for (int e = 0; e < 4; ++e) Ct[(wr + i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][j][e] + bj;
+				for (int e = 0; e < 4; ++e) Ct[(i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][e] + bias;
 		}
 		__syncthreads();
 		const float *crow = Ct + orow * kFCStride + opart * 16;
@@ -415,8 +408,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 			}
 		}
 		__syncthreads();  // the output tile is read: Wt is free for tile t + 2
-		bias[0] = bnext[0];
-		bias[1] = bnext[1];
+		bias = bnext;
 	}
 	// merge the four parts of each ray (value descending, index ascending)
 	top_merge(v, ix, 1);
