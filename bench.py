@@ -71,11 +71,22 @@ def cpu_baseline(tris, rays, target_s=10.0):
             bvh.trace(rays[:count], oracle.DFS, threads)
         runs.append(time.perf_counter() - t0)
     dt = float(np.median(runs))
-    return {"value": round(reps * count / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    port = {"value": round(reps * count / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"median of 3 runs, each {reps} pass(es) over the first {count} of the 8,388,608 rays, same "
                       f"scene, oracle depth-first mode (the reference CPU backend restated), {threads} threads, "
                       f"{dt:.2f} s per run",
             "host": host_info()}
+    if not oracle.have_ref():
+        return port
+    # the reference's own CPU backend, compiled from its sources (its BVH2 build, traversal and
+    # hardware_concurrency() thread split), on the same scene and every ray
+    sec = oracle.ref_time(tris, rays, 3)
+    return {"value": round(len(rays) / sec / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "sample": f"best of 3 nearest_hits<t, primitive_id> calls over all 8,388,608 rays after set_tris, "
+                      f"same scene, the reference CPU backend compiled from its sources (threads: "
+                      f"hardware_concurrency() = {os.cpu_count()}, on the {threads} CPUs this process may use), "
+                      f"{sec:.2f} s per call",
+            "port_value": port["value"], "port_sample": port["sample"], "host": port["host"]}
 
 
 def fp64_report(tris, rays, rec):

@@ -173,6 +173,16 @@ def ref_trace(tris, rays):
     return out
 
 
+def ref_time(tris, rays, reps=3):
+    """Seconds of the compiled reference's nearest_hits<t, primitive_id> over the rays (its own
+    CPU backend and thread split), best of reps calls after set_tris."""
+    tris, rays = _f32(tris, 9), _f32(rays, 6)
+    s = ref().ref_time_trace(_p(tris), len(tris), _p(rays), len(rays), int(reps))
+    if s < 0:
+        raise RuntimeError("reference trace returned a short record array")
+    return s
+
+
 def ref_layout(mask):
     out = np.zeros(9, np.int32)
     if ref().ref_hitreg_layout(mask, _p(out)):
