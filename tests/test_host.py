@@ -182,6 +182,22 @@ def test_bench_refuses_more_than_one_gpu():
     assert "measures one GPU" in r.stderr
 
 
+def test_bench_cpu_baseline_leg():
+    """The bench's CPU leg on a small scene: the compiled reference where it was built (else the
+    restatement), a positive rate, the restatement's rate alongside, and the host recorded."""
+    sys.path.insert(0, ROOT)
+    import bench
+    import oracle
+    tris = scenes.scene(0)
+    rays = scenes.rays(0)[:20000]
+    out = bench.cpu_baseline(tris, rays, target_s=0.3)
+    assert out["unit"] == "Mrays/s" and out["value"] > 0 and out["cores"] >= 1
+    assert out["kind"] == ("reference" if oracle.have_ref() else "port")
+    assert out["host"]["affinity"] == bench.host_threads()
+    if out["kind"] == "reference":
+        assert out["port_value"] > 0
+
+
 REF_TREE = "/root/reference"
 
 
