@@ -321,7 +321,7 @@ int orc_mt(const float *tri, const float *r, float *t, float *u, float *v) {
 }
 
 /* Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection",
- * JCGT 2(1), 2013), the north star's optional leaf test beside the reference's Moller-Trumbore.
+ * JCGT 2(1), 2013), an optional leaf test beside the reference's Moller-Trumbore.
  * The reference has no such test; the semantics kept from core.hpp:27-65 are: both windings
  * accepted, t returned without a range check, (u, v) the weights of vertices 1 and 2. Steps:
  *   kz = axis of the largest |d| (first on ties), kx = kz + 1, ky = kx + 1 (mod 3), kx and ky
