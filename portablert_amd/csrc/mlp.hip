@@ -168,19 +168,18 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 //
 // One workgroup (4 waves) takes 64 rays through the whole network: encoding, the three hidden
 // layers and the 4097 output logits, keeping the activations in LDS (X, 64 x 256 bf16) and
-// streaming W^T tiles of 64 output neurons (W, prefetched into registers one tile ahead) from
-// L2. In the hidden layers each wave computes a 32 x 32 part of every 64 x 64 tile with
+// streaming W^T tiles of 64 output neurons (prefetched into registers one tile ahead) from L2.
+// In the hidden layers each wave computes a 32 x 32 part of every 64 x 64 tile with
 // v_mfma_f32_16x16x32_bf16 over K in steps of 32. For the output layer every wave loads all 64
-// rows of the last hidden layer into registers and computes 16 of each tile's 64 columns, so
-// the 65 output tiles read only W from LDS, each W fragment once per 64 rows. Every output tile
-// goes through LDS (over its W buffer; W is double-buffered in X's space once X's rows are in
-// registers) to four row-owner threads per ray (16 columns each) that keep a running top-4
-// (value descending, index ascending); no logit leaves the chip. The workgroup uses 66 KB of
-// LDS, so two share a CU: one's MFMAs overlap the other's top-4 scan.
+// rows of the last hidden layer into registers and owns 16 of each tile's 64 output neurons: it
+// fetches and stages only its own 16 rows of W^T (in its slice of two tile buffers, one of them
+// in X's space), and passes its 64 x 16 logits through the same slice to its 64 lanes, one ray
+// each, which keep a running top-4 (value descending, index ascending). No wave reads another's
+// slice, so the 65 output tiles run without a barrier; the four waves' lists of a ray are merged
+// at the end, and no logit leaves the chip. The workgroup uses 66 KB of LDS, so two share a CU.
 constexpr int kFRows = 64;     // rays per workgroup
 constexpr int kFCols = 64;     // output neurons per W tile
 constexpr int kFStride = 264;  // bf16 row stride of X and W (16-byte reads without bank conflicts)
-constexpr int kFCStride = 68;  // fp32 row stride of the output tile over W
 
 struct FusedArgs {
 	const float *rays;
@@ -329,21 +328,39 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 #pragma unroll
 		for (int kk = 0; kk < kMlpHidden / kBK; ++kk)
 			ar[i][kk] = *reinterpret_cast<const bf16x8 *>(X + (i * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+	// Output tiles without barriers: every wave fetches, stages and reads only its own 16 rows of
+	// each W^T tile (columns oc .. oc + 15 of the 64 output neurons), in its own slice of the two
+	// tile buffers, and passes its 64 x 16 block of logits through that slice (free once its
+	// MFMAs have read it) to its 64 lanes, one row each: lane r keeps the running top-4 of row r
+	// over this wave's columns. The four waves' lists of a row are merged at the end.
+	constexpr int kSliceStride = 20;  // fp32 row stride of a wave's 64 x 16 logit block (16-byte rows, no bank conflicts)
+	auto slice = [&](uint16_t *buf) { return buf + oc * kFStride; };
+	auto fetch_own = [&](int t, u32x4 q[8]) {
+		// 16 rows x 512 bytes: chunk c = lane + 64 j is row c / 32, 16-byte column group c % 32
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const int c = lane + 64 * j;
+			q[j] = *reinterpret_cast<const u32x4 *>(a.w[3] + static_cast<size_t>(t * kFCols + oc + c / 32) * kMlpHidden +
+			                                        (c % 32) * 8);
+		}
+	};
+	auto store_own = [&](uint16_t *buf, const u32x4 q[8]) {
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const int c = lane + 64 * j;
+			*reinterpret_cast<u32x4 *>(slice(buf) + (c / 32) * kFStride + (c % 32) * 8) = q[j];
+		}
+	};
 	{
-		u32x4 p0[kMlpHidden / 32];
-		tile_fetch<kMlpHidden>(a.w[3], 0, tid, p0);
-		tile_store<kMlpHidden>(W, tid, p0);
+		u32x4 p0[8];
+		fetch_own(0, p0);
+		__syncthreads();  // every wave has its rows of X in registers: X's space becomes a tile buffer
+		store_own(W, p0);
 	}
-	__syncthreads();
-	// Each output tile (plus bias) goes through LDS (over W) to four row-owner threads per ray
-	// (16 columns each), which keep a running top-4. A thread sees its columns in ascending
-	// order, so a strict comparison keeps the lower index first among equal logits.
-	const int orow = tid >> 2, opart = tid & 3;
 	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
 	constexpr int kOutTiles = (kMlpOut + kFCols - 1) / kFCols;  // the last one holds the miss logit
-	// bias of this thread's two columns, loaded one tile ahead (padding columns: -inf, never
-	// taken into a top-4)
+	// bias of this lane's column, loaded one tile ahead (padding columns: -inf, never taken)
 	auto bias_of = [&](int t) {
 		const int col = t * kFCols + oc + (lane & 15);
 		return col < kMlpOut ? a.b[3][col] : -__builtin_inff();
@@ -351,46 +368,41 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	float bias = bias_of(0);
 	for (int t = 0; t < kOutTiles; ++t) {
 		const float bnext = bias_of(t + 1);
-		// tile t is in Wt; the next goes to the other buffer (X's, free once its rows are in
-		// registers) during this tile's MFMAs. The output tile goes through LDS over Wt.
 		uint16_t *Wt = (t & 1) ? X : W, *Wn = (t & 1) ? W : X;
-		float *Ct = reinterpret_cast<float *>(Wt);
-		u32x4 pre[kMlpHidden / 32];
-		tile_fetch<kMlpHidden>(a.w[3], ((t + 1) % kOutTiles) * kFCols, tid, pre);
+		u32x4 pre[8];
+		fetch_own((t + 1) % kOutTiles, pre);
 		f32x4 acc[4];
 #pragma unroll
 		for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
 		for (int kk = 0; kk < kMlpHidden / kBK; ++kk) {
-			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(Wt + (oc + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(slice(Wt) + (lane & 15) * kFStride + kk * kBK + (lane >> 4) * 8);
 #pragma unroll
 			for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr, acc[i], 0, 0, 0);
 		}
-		tile_store<kMlpHidden>(Wn, tid, pre);
-		__syncthreads();  // every wave is done reading Wt; the next tile is stored
-		{
-			const int col = oc + (lane & 15);
+		store_own(Wn, pre);
+		// this wave's logits into its slice of Wt (its MFMAs have read it; LDS operations of one
+		// wave complete in order)
+		float *C = reinterpret_cast<float *>(slice(Wt));
 #pragma unroll
-			for (int i = 0; i < 4; ++i)
+		for (int i = 0; i < 4; ++i)
 #pragma unroll
-				for (int e = 0; e < 4; ++e) Ct[(i * 16 + (lane >> 4) * 4 + e) * kFCStride + col] = acc[i][e] + bias;
-		}
-		__syncthreads();
-		const float *crow = Ct + orow * kFCStride + opart * 16;
-		const int nbase = t * kFCols + opart * 16;
-		// candidates: above this thread's 4th best, and not below the row's bound (the largest
-		// 4th best of the row's four threads: the row's final 4th best is at least that)
-		const float tau1 = fmaxf(v[3], __shfl_xor(v[3], 1));
-		const float tau = fmaxf(tau1, __shfl_xor(tau1, 2));
+			for (int e = 0; e < 4; ++e) C[(i * 16 + (lane >> 4) * 4 + e) * kSliceStride + (lane & 15)] = acc[i][e] + bias;
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		// lane r scans row r's 16 logits in column order (a strict comparison keeps the lower
+		// index first; held entries come from earlier tiles)
+		const float *crow = C + lane * kSliceStride;
+		const int nbase = t * kFCols + oc;
 		uint32_t cand = 0;
 #pragma unroll
 		for (int c4 = 0; c4 < 4; ++c4) {
 			const float4 q = *reinterpret_cast<const float4 *>(crow + c4 * 4);
 			const float xs[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-			for (int e = 0; e < 4; ++e) cand |= (xs[e] > v[3] && xs[e] >= tau) ? (1u << (c4 * 4 + e)) : 0u;
+			for (int e = 0; e < 4; ++e) cand |= xs[e] > v[3] ? (1u << (c4 * 4 + e)) : 0u;
 		}
-		// insert them in column order (a strict comparison keeps the lower index first)
 		while (cand) {
 			const int c = __builtin_ctz(cand);
 			cand &= cand - 1u;
@@ -407,10 +419,25 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 				}
 			}
 		}
-		__syncthreads();  // the output tile is read: Wt is free for tile t + 2
+		__builtin_amdgcn_wave_barrier();  // this tile's reads of the slice before the next tile's writes
 		bias = bnext;
 	}
-	// merge the four parts of each ray (value descending, index ascending)
+	// merge the four waves' lists of each row (value descending, index ascending)
+	__syncthreads();
+	float *mv = reinterpret_cast<float *>(W);
+	uint32_t *mi = reinterpret_cast<uint32_t *>(W) + 4 * 64 * 4;
+#pragma unroll
+	for (int k = 0; k < 4; ++k) {
+		mv[(wave * 64 + lane) * 4 + k] = v[k];
+		mi[(wave * 64 + lane) * 4 + k] = ix[k];
+	}
+	__syncthreads();
+	const int orow = tid >> 2, opart = tid & 3;
+#pragma unroll
+	for (int k = 0; k < 4; ++k) {
+		v[k] = mv[(opart * 64 + orow) * 4 + k];
+		ix[k] = mi[(opart * 64 + orow) * 4 + k];
+	}
 	top_merge(v, ix, 1);
 	top_merge(v, ix, 2);
 	if (opart == 0 && r0 + orow < a.rows) {
