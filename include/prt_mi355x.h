@@ -59,7 +59,7 @@ enum {
 	                             (Woop, Benthin, Wald 2013; plain traversal only, no distance
 	                             culling) */
 	PRT_OPT_SEEDS = 17        /* learned traversal: the best 1..4 predicted clusters searched
-	                             first (default 4) */
+	                             first (default 1, the fastest measured) */
 };
 
 /* Statistics (prt_mi355x_get_stat). */

@@ -132,7 +132,7 @@ struct TraceParams {
 	int refill = 8;           // load new rays once at least this many lanes of a wave are idle
 	int count_stats = 0;      // count node visits and triangle tests
 	int leaf_test = 0;        // 0 Moller-Trumbore (reference), 1 watertight (no distance culling)
-	int n_seeds = 4;          // learned traversal: predicted clusters searched first (1..4)
+	int n_seeds = 1;          // learned traversal: predicted clusters searched first (1..4)
 	float margin_rel = 0x1p-10f;
 	float margin_abs = 0.0f;  // absolute margin in scene units (set from the scene size)
 };

@@ -63,8 +63,9 @@ struct KArgs {
 	const float2 *scene_info;
 };
 
-// kWatertight: the plain traversal with the watertight leaf test (wt_candidate), no culling
-enum : int { kPlain = 0, kSeeded = 1, kBatch = 2, kWatertight = 3 };
+// kWatertight: the plain traversal with the watertight leaf test (wt_candidate), no culling.
+// kSeeded searches the best predicted cluster first, kSeededMulti the best 2..4.
+enum : int { kPlain = 0, kSeeded = 1, kBatch = 2, kWatertight = 3, kSeededMulti = 4 };
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
@@ -438,15 +439,17 @@ __device__ __forceinline__ void test_leaf(const TriRec *tris, uint32_t entry, Ra
 // (4 KB per wave): 8 for the plain mode, i.e. at most 64 VGPRs; the traversal is partly
 // latency-bound (dependent node fetches) and the extra waves hide more of it than the few
 // spilled registers cost. The seeded and batch modes carry more state per lane (seed clusters,
-// scene tiles) and spill about twice as much at 64 VGPRs: 7 waves (72 VGPRs) is faster for
-// both. A 16-entry stack allows 5 waves by LDS alone. The fp32 node test needs about 100 VGPRs
+// scene tiles) and spill more at 64 VGPRs: they run at 7 waves (72 VGPRs). Spills inside the
+// loop are costly here: the multi-seed state (three more entries, their flags) spills 14
+// registers at 72 VGPRs, and the one-seed mode, which carries one entry, runs 10% faster with
+// 2. A 16-entry stack allows 5 waves by LDS alone. The fp32 node test needs about 100 VGPRs
 // and is left to the compiler (1 = no constraint).
 template <class Node, int S, int MODE>
 constexpr int kWavesPerEu = std::is_same<Node, NodeQ>::value ? (S == 8 ? (MODE == kPlain ? 8 : 7) : 5) : 1;
 
 template <class Node, int S, int B, bool COUNT, int MODE, bool NC>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<Node, S, MODE>))) void k_trace(KArgs a) {
-	constexpr bool SEEDED = MODE == kSeeded, BATCH = MODE == kBatch;
+	constexpr bool MULTI = MODE == kSeededMulti, SEEDED = MODE == kSeeded || MULTI, BATCH = MODE == kBatch;
 	__shared__ uint2 lds[S * B];
 	const Node *nodes = static_cast<const Node *>(a.nodes);
 	const uint32_t lane = threadIdx.x & 63u;
@@ -462,11 +465,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 	bool active = false;       // lane holds a ray
 	uint32_t cur = kInvalid;   // next entry to process (node or leaf)
 	uint32_t pend = kInvalid;  // parked leaf, tested in the next triangle phase
-	// learned seeding: the seed cluster entries, which of them were searched, and whether the
-	// root (pushed below the seeds) has been reached
+	// learned seeding: the entries of the seed clusters (sd[0] the best) and, with several
+	// seeds, which of sd[1..3] have been taken from the stack
 	uint32_t sd[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
 	uint32_t seen = 0u;
-	bool root_phase = true;
 	// wave-uniform ray chunk [chunk_next, chunk_end) reserved with one atomic
 	uint64_t chunk_next = 0, chunk_end = 0;
 	uint32_t chunk_scene = 0;
@@ -519,28 +521,43 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				pend = kInvalid;
 				st.sp = 0;
 				if (SEEDED && a.n_nodes > 0) {
-					// the root goes to the bottom of the stack, the predicted clusters above it
-					// (best on top): their subtrees are intersected exactly first and seed the
-					// culling distance; the root traversal afterwards covers every triangle
-					// except those of the clusters already searched
-					st.push(0u, sort_key(-3.402823466e38f, 0));
-					cur = kInvalid;
-					root_phase = false;
-					seen = 0u;
-					const uint4 s4 = reinterpret_cast<const uint4 *>(a.seeds)[ray_idx];
-					const uint32_t sc[4] = {s4.w, s4.z, s4.y, s4.x};
+					// The lane starts at the best predicted cluster's entry instead of the root,
+					// which waits at the bottom of the stack; with several seeds the others sit
+					// above the root (the better on top) with their box entry distances. The
+					// clusters' subtrees are intersected exactly first and seed the culling
+					// distance; the root traversal afterwards covers every triangle except those
+					// of the clusters already searched. The best cluster's box is not tested:
+					// the leaf test is exact, so a ray that misses it only visits its entry.
+					const uint4 s4 = reinterpret_cast<const uint4 *>(a.seeds)[ray_idx];  // best first
+					sd[0] = s4.x < a.n_clusters ? a.cl_entry[s4.x] : 0u;
+					if (sd[0] == 0u) sd[0] = kInvalid;  // no seed, or the cluster is the whole tree
+					uint32_t key[4] = {};
+					if (MULTI) {
+						seen = 0u;
+						const uint32_t sc[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
-					for (int q = 0; q < 4; ++q) {
-						sd[q] = kInvalid;
-						if (sc[q] < a.n_clusters && q >= 4 - a.n_seeds) {  // the best n_seeds only
-							const float *bx = a.cl_box + 6ull * sc[q];
-							const float lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
-							const float tmin = child_slab(r, lo, hi, __builtin_inff());
-							if (!__builtin_isnan(tmin)) {
-								sd[q] = a.cl_entry[sc[q]];
-								st.push(sd[q], sort_key(tmin, 0));
+						for (int q = 1; q < 4; ++q) {
+							sd[q] = kInvalid;
+							if (q < a.n_seeds && sc[q] < a.n_clusters) {
+								const float *bx = a.cl_box + 6ull * sc[q];
+								const float lo[3] = {bx[0], bx[1], bx[2]}, hi[3] = {bx[3], bx[4], bx[5]};
+								const float tmin = child_slab(r, lo, hi, __builtin_inff());
+								const uint32_t ce = a.cl_entry[sc[q]];
+								if (ce != 0u && !__builtin_isnan(tmin)) {
+									sd[q] = ce;
+									key[q] = sort_key(tmin, 0);
+								}
 							}
 						}
+					}
+					if (sd[0] != kInvalid || (MULTI && (sd[1] & sd[2] & sd[3]) != kInvalid)) {
+						st.push(0u, sort_key(-3.402823466e38f, 0));
+						if (MULTI) {
+#pragma unroll
+							for (int q = 3; q >= 1; --q)
+								if (sd[q] != kInvalid) st.push(sd[q], key[q]);
+						}
+						cur = sd[0];  // kInvalid: the next pop takes the best pushed cluster
 					}
 				}
 			}
@@ -556,12 +573,18 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 		if (!active) continue;
 
 		// clusters searched in the seed phase are not searched again: their entries are dropped
-		// when the root traversal takes them (from the stack or as a node's nearest child)
+		// when the root traversal takes them (from the stack or as a node's nearest child). The
+		// clusters are disjoint subtrees, so a cluster's entry is never met inside another
+		// cluster's search; the best one is never on the stack, so meeting its entry always
+		// means the root traversal reached it. A pushed cluster counts once it has been taken
+		// from the stack (one culled there is traversed if the root traversal reaches it).
 		auto searched = [&](uint32_t e) {
-			if (!SEEDED || !root_phase) return false;
-			bool s = false;
+			if (!SEEDED) return false;
+			bool s = e == sd[0];
+			if (MULTI) {
 #pragma unroll
-			for (int q = 0; q < 4; ++q) s |= ((seen >> q) & 1u) && e == sd[q];
+				for (int q = 1; q < 4; ++q) s |= ((seen >> q) & 1u) && e == sd[q];
+			}
 			return s;
 		};
 
@@ -574,13 +597,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				const uint2 e = st.pop();
 				if (e.y <= klim) {
 					cur = e.x;
-					if (SEEDED && !root_phase) {
-						// a cluster popped here is searched completely before anything below it
-						if (cur == 0u) root_phase = true;
-#pragma unroll
-						for (int q = 0; q < 4; ++q) seen |= (cur == sd[q] && cur != 0u) ? (1u << q) : 0u;
-					} else if (searched(cur)) {
+					if (searched(cur)) {
 						cur = kInvalid;
+					} else if (MULTI) {
+						// a pushed cluster taken here is searched before anything below it
+#pragma unroll
+						for (int q = 1; q < 4; ++q) seen |= cur == sd[q] ? (1u << q) : 0u;
 					}
 				} else if (COUNT) {
 					++c_culled;
@@ -684,6 +706,7 @@ void launch_m(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
 template <class Node, int S, int B>
 hipError_t launch(const KArgs &a, uint32_t grid, bool count, hipStream_t st) {
 	if (a.tiles) launch_m<Node, S, B, kBatch>(a, grid, count, st);
+	else if (a.seeds && a.n_seeds > 1) launch_m<Node, S, B, kSeededMulti>(a, grid, count, st);
 	else if (a.seeds) launch_m<Node, S, B, kSeeded>(a, grid, count, st);
 	else if (a.leaf_test == 1) launch_m<Node, S, B, kWatertight>(a, grid, count, st);
 	else launch_m<Node, S, B, kPlain>(a, grid, count, st);

@@ -96,13 +96,19 @@ def test_learned_traversal_identical_to_plain(gpu_ctx, room):
     a = torch.zeros(nf * fsize, dtype=torch.uint8, device="cuda")
     b = torch.zeros_like(a)
     torch.cuda.synchronize()  # the library works on its own stream
-    for cull in (1, 0):
-        gpu_ctx.set_option("cull", cull)
-        gpu_ctx.nearest_hits_device(d_test.data_ptr(), nf, full, a.data_ptr())
-        prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), nf, full, b.data_ptr())
-        gpu_ctx.synchronize()
-        assert torch.equal(a, b), f"learned differs from plain (all fields, cull {cull})"
-    gpu_ctx.set_option("cull", 1)
+    try:
+        for cull in (1, 0):
+            gpu_ctx.set_option("cull", cull)
+            gpu_ctx.nearest_hits_device(d_test.data_ptr(), nf, full, a.data_ptr())
+            for seeds in (1, 4):  # one seed, and the several-seed kernel
+                gpu_ctx.set_option("seeds", seeds)
+                b.zero_()
+                prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), nf, full, b.data_ptr())
+                gpu_ctx.synchronize()
+                assert torch.equal(a, b), f"learned differs from plain (all fields, cull {cull}, {seeds} seeds)"
+    finally:
+        gpu_ctx.set_option("cull", 1)
+        gpu_ctx.set_option("seeds", 1)
     got = out_learn.cpu().numpy().view(prt.hit_dtype(mask))
     idx = np.random.default_rng(0).choice(len(test), 8192, replace=False)
     want = oracle.trace(room, test[idx], oracle.LOWEST_ID)
@@ -130,7 +136,36 @@ def test_seed_count_keeps_records(gpu_ctx, room):
             gpu_ctx.synchronize()
             assert torch.equal(plain, learn), f"learned with {k} seeds differs from plain"
     finally:
-        gpu_ctx.set_option("seeds", 4)
+        gpu_ctx.set_option("seeds", 1)
+
+
+def test_untrained_seeds_keep_records(gpu_ctx, room):
+    """With an untrained network the predicted clusters are mostly wrong (many rays miss the
+    best cluster's box, which the one-seed traversal enters without testing it): records still
+    equal the plain traversal's byte for byte, for 1 and 4 seeds, with and without culling."""
+    import torch
+    prt.learned_setup(gpu_ctx, 11)
+    try:
+        test = scenes.rays(4, stream=2, first=65536, count=131072)
+        d_test = torch.from_numpy(test).cuda()
+        mask = prt.mask_of(prt.ALL_TAGS)
+        size, _ = prt.layout(mask)
+        plain = torch.zeros(len(test) * size, dtype=torch.uint8, device="cuda")
+        learn = torch.zeros_like(plain)
+        torch.cuda.synchronize()
+        for cull in (1, 0):
+            gpu_ctx.set_option("cull", cull)
+            gpu_ctx.nearest_hits_device(d_test.data_ptr(), len(test), mask, plain.data_ptr())
+            for k in (1, 4):
+                gpu_ctx.set_option("seeds", k)
+                learn.zero_()
+                prt.nearest_hits_learned_device(gpu_ctx, d_test.data_ptr(), len(test), mask, learn.data_ptr())
+                gpu_ctx.synchronize()
+                assert torch.equal(plain, learn), f"untrained, {k} seeds, cull {cull}: differs from plain"
+    finally:
+        gpu_ctx.set_option("cull", 1)
+        gpu_ctx.set_option("seeds", 1)
+        prt.learned_setup(gpu_ctx, 7)
 
 
 def test_refit_requires_learned_setup_again(gpu_ctx, room):
