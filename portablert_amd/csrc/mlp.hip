@@ -176,7 +176,9 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 // in X's space), and passes its 64 x 16 logits through the same slice to its 64 lanes, one ray
 // each, which keep a running top-4 (value descending, index ascending). No wave reads another's
 // slice, so the 65 output tiles run without a barrier; the four waves' lists of a ray are merged
-// at the end, and no logit leaves the chip. The workgroup uses 66 KB of LDS, so two share a CU.
+// at the end, and no logit leaves the chip. The tiles are software-pipelined: a wave issues tile
+// t + 1's MFMAs interleaved with the compares of tile t's logits against its running bound.
+// The workgroup uses 66 KB of LDS, so two share a CU.
 constexpr int kFRows = 64;     // rays per workgroup
 constexpr int kFCols = 64;     // output neurons per W tile
 constexpr int kFStride = 264;  // bf16 row stride of X and W (16-byte reads without bank conflicts)
@@ -334,15 +336,17 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	// MFMAs have read it) to its 64 lanes, one row each: lane r keeps the running top-4 of row r
 	// over this wave's columns. The four waves' lists of a row are merged at the end.
 	constexpr int kSliceStride = 20;  // fp32 row stride of a wave's 64 x 16 logit block (16-byte rows, no bank conflicts)
-	auto slice = [&](uint16_t *buf) { return buf + oc * kFStride; };
+	auto slice = [&](auto *buf) { return buf + oc * kFStride; };
+	// this wave's 16 rows of tile t's W^T, 16 x 512 bytes: chunk c = lane + 64 j is row c / 32,
+	// 16-byte column group c % 32
+	auto w_chunk = [&](int t, int j) {
+		const int c = lane + 64 * j;
+		return *reinterpret_cast<const u32x4 *>(a.w[3] + static_cast<size_t>(t * kFCols + oc + c / 32) * kMlpHidden +
+		                                        (c % 32) * 8);
+	};
 	auto fetch_own = [&](int t, u32x4 q[8]) {
-		// 16 rows x 512 bytes: chunk c = lane + 64 j is row c / 32, 16-byte column group c % 32
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const int c = lane + 64 * j;
-			q[j] = *reinterpret_cast<const u32x4 *>(a.w[3] + static_cast<size_t>(t * kFCols + oc + c / 32) * kMlpHidden +
-			                                        (c % 32) * 8);
-		}
+		for (int j = 0; j < 8; ++j) q[j] = w_chunk(t, j);
 	};
 	auto store_own = [&](uint16_t *buf, const u32x4 q[8]) {
 #pragma unroll
@@ -351,50 +355,43 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 			*reinterpret_cast<u32x4 *>(slice(buf) + (c / 32) * kFStride + (c % 32) * 8) = q[j];
 		}
 	};
-	{
-		u32x4 p0[8];
-		fetch_own(0, p0);
-		__syncthreads();  // every wave has its rows of X in registers: X's space becomes a tile buffer
-		store_own(W, p0);
-	}
 	float v[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 	uint32_t ix[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
 	constexpr int kOutTiles = (kMlpOut + kFCols - 1) / kFCols;  // the last one holds the miss logit
-	// bias of this lane's column, loaded one tile ahead (padding columns: -inf, never taken)
+	// bias of this lane's column (padding columns: -inf, never taken)
 	auto bias_of = [&](int t) {
 		const int col = t * kFCols + oc + (lane & 15);
-		return col < kMlpOut ? a.b[3][col] : -__builtin_inff();
+		const float b = a.b[3][col < kMlpOut ? col : kMlpOut - 1];  // no branch around the load
+		return col < kMlpOut ? b : -__builtin_inff();
 	};
-	float bias = bias_of(0);
-	for (int t = 0; t < kOutTiles; ++t) {
-		const float bnext = bias_of(t + 1);
-		uint16_t *Wt = (t & 1) ? X : W, *Wn = (t & 1) ? W : X;
-		u32x4 pre[8];
-		fetch_own((t + 1) % kOutTiles, pre);
-		f32x4 acc[4];
+	auto mma = [&](const uint16_t *buf, f32x4 acc[4]) {
 #pragma unroll
 		for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
 		for (int kk = 0; kk < kMlpHidden / kBK; ++kk) {
-			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(slice(Wt) + (lane & 15) * kFStride + kk * kBK + (lane >> 4) * 8);
+			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(slice(buf) + (lane & 15) * kFStride + kk * kBK + (lane >> 4) * 8);
 #pragma unroll
 			for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bfr, acc[i], 0, 0, 0);
 		}
-		store_own(Wn, pre);
-		// this wave's logits into its slice of Wt (its MFMAs have read it; LDS operations of one
-		// wave complete in order)
-		float *C = reinterpret_cast<float *>(slice(Wt));
+	};
+	// this wave's logits into its slice of buf, whose W the MFMAs that made them have read (LDS
+	// operations of one wave complete in order)
+	auto put_logits = [&](uint16_t *buf, const f32x4 acc[4], float bias) {
+		float *C = reinterpret_cast<float *>(slice(buf));
 #pragma unroll
 		for (int i = 0; i < 4; ++i)
 #pragma unroll
 			for (int e = 0; e < 4; ++e) C[(i * 16 + (lane >> 4) * 4 + e) * kSliceStride + (lane & 15)] = acc[i][e] + bias;
+	};
+	auto sync_wave = [&]() {
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-		// lane r scans row r's 16 logits in column order (a strict comparison keeps the lower
-		// index first; held entries come from earlier tiles)
-		const float *crow = C + lane * kSliceStride;
-		const int nbase = t * kFCols + oc;
+	};
+	// lane r compares row r's 16 logits of tile t with its 4th best (a strict comparison keeps
+	// the lower index first; held entries come from earlier tiles)
+	auto scan_mask = [&](const uint16_t *buf) {
+		const float *crow = reinterpret_cast<const float *>(slice(buf)) + lane * kSliceStride;
 		uint32_t cand = 0;
 #pragma unroll
 		for (int c4 = 0; c4 < 4; ++c4) {
@@ -403,6 +400,11 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 #pragma unroll
 			for (int e = 0; e < 4; ++e) cand |= xs[e] > v[3] ? (1u << (c4 * 4 + e)) : 0u;
 		}
+		return cand;
+	};
+	auto insert = [&](const uint16_t *buf, uint32_t cand, int t) {
+		const float *crow = reinterpret_cast<const float *>(slice(buf)) + lane * kSliceStride;
+		const int nbase = t * kFCols + oc;
 		while (cand) {
 			const int c = __builtin_ctz(cand);
 			cand &= cand - 1u;
@@ -419,9 +421,72 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 				}
 			}
 		}
-		__builtin_amdgcn_wave_barrier();  // this tile's reads of the slice before the next tile's writes
-		bias = bnext;
+	};
+	// Software pipeline over the output tiles: while the matrix core computes tile t + 1, the
+	// wave scans tile t's logits. Buffer roles alternate: at the top of iteration t, `cur` is
+	// free (its W was read by tile t's MFMAs, whose results are in acc) and `nxt` holds tile
+	// t + 1's W.
+	f32x4 acc[4];
+	uint16_t *cur = W, *nxt = X;
+	{
+		u32x4 p0[8];
+		fetch_own(0, p0);
+		__syncthreads();  // every wave has its rows of X in registers: X's space becomes a tile buffer
+		store_own(cur, p0);
+		fetch_own(1, p0);
+		store_own(nxt, p0);
+		sync_wave();
+		mma(cur, acc);
 	}
+	float bias = bias_of(0);
+	for (int t = 0; t < kOutTiles - 1; ++t) {
+		put_logits(cur, acc, bias);
+		bias = bias_of(t + 1);
+		sync_wave();
+		// Tile t + 1's MFMAs with tile t's scan in hand-ordered steps (a scheduling barrier after
+		// each keeps the order): the four logit reads and the first two B fragment reads, then per
+		// K step the B fragment read two steps ahead, four MFMAs, one W prefetch load and two of
+		// the sixteen compares, which issue while the matrix core works.
+		const float *crow = reinterpret_cast<const float *>(slice(cur)) + lane * kSliceStride;
+		float lf[16];
+#pragma unroll
+		for (int c4 = 0; c4 < 4; ++c4) {
+			const float4 q = *reinterpret_cast<const float4 *>(crow + c4 * 4);
+			lf[c4 * 4] = q.x;
+			lf[c4 * 4 + 1] = q.y;
+			lf[c4 * 4 + 2] = q.z;
+			lf[c4 * 4 + 3] = q.w;
+		}
+		const uint16_t *bsrc = slice(nxt) + (lane & 15) * kFStride + (lane >> 4) * 8;
+		bf16x8 bq[kMlpHidden / kBK];
+		bq[0] = *reinterpret_cast<const bf16x8 *>(bsrc);
+		bq[1] = *reinterpret_cast<const bf16x8 *>(bsrc + kBK);
+#pragma unroll
+		for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+		__builtin_amdgcn_sched_barrier(0);
+		u32x4 pre[8];
+		// tile t + 2's W rows of this wave: chunk kk of w_chunk is 2 kk rows further
+		const uint16_t *wsrc = a.w[3] + static_cast<size_t>(((t + 2) % kOutTiles) * kFCols + oc + (lane >> 5)) * kMlpHidden +
+		                       (lane & 31) * 8;
+		uint32_t cand = 0;
+		const float thr = v[3];
+#pragma unroll
+		for (int kk = 0; kk < kMlpHidden / kBK; ++kk) {
+			if (kk + 2 < kMlpHidden / kBK) bq[kk + 2] = *reinterpret_cast<const bf16x8 *>(bsrc + (kk + 2) * kBK);
+			pre[kk] = *reinterpret_cast<const u32x4 *>(wsrc + 2 * kk * kMlpHidden);
+#pragma unroll
+			for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[i][kk], bq[kk], acc[i], 0, 0, 0);
+			cand |= (lf[2 * kk] > thr ? (1u << (2 * kk)) : 0u) | (lf[2 * kk + 1] > thr ? (1u << (2 * kk + 1)) : 0u);
+			__builtin_amdgcn_sched_barrier(0);
+		}
+		insert(cur, cand, t);
+		sync_wave();
+		store_own(cur, pre);  // tile t + 2's W into the buffer tile t's logits have left
+		uint16_t *tmp = cur; cur = nxt; nxt = tmp;
+	}
+	put_logits(cur, acc, bias);
+	sync_wave();
+	insert(cur, scan_mask(cur), kOutTiles - 1);
 	// merge the four waves' lists of each row (value descending, index ascending)
 	__syncthreads();
 	float *mv = reinterpret_cast<float *>(W);
