@@ -475,6 +475,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 	bool exhausted = false;
 	unsigned long long c_nodes = 0, c_tris = 0, c_leaves = 0, c_nphase = 0, c_tphase = 0, c_culled = 0;
 
+	// Issue priority: 1 while the wave may still reserve rays, 0 once the work is used up. At the
+	// end of a launch the waves that still hold reserved rays (the last to finish) then win the
+	// issue slots over waves draining their last rays. The SIMD's arbiter otherwise favours the
+	// older waves, and the younger ones, which trace about half as many rays, end the launch.
+	__builtin_amdgcn_s_setprio(1);
 	while (true) {
 		// ---- refill finished lanes from the wave's chunk (one ballot when none is idle); the
 		// ray loads are amortised over at least `refill` lanes unless the whole wave is idle
@@ -493,6 +498,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				} else {
 					exhausted = true;
 					chunk_next = chunk_end = 0;
+					__builtin_amdgcn_s_setprio(0);
 				}
 			} else if (chunk_next >= chunk_end) {
 				uint64_t base = 0;
@@ -501,7 +507,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<N
 				base = uniform64(base);
 				chunk_next = base;
 				chunk_end = base + a.chunk < a.n_rays ? base + a.chunk : a.n_rays;
-				if (base >= a.n_rays) exhausted = true;
+				if (base >= a.n_rays) {
+					exhausted = true;
+					__builtin_amdgcn_s_setprio(0);
+				}
 			}
 			const uint64_t avail = chunk_end > chunk_next ? chunk_end - chunk_next : 0ull;
 			// idle lanes below this one
