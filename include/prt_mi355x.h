@@ -42,7 +42,7 @@ enum {
 	                             an ill-conditioned triangle never culled */
 	PRT_OPT_LEAF_SIZE = 3,    /* max triangles per leaf, 1..4 (default 3); takes effect at set_tris */
 	PRT_OPT_BLOCK = 4,        /* traversal workgroup size: 64, 128 (default), 256 */
-	PRT_OPT_WAVES_PER_CU = 5, /* persistent waves per CU (default 32) */
+	PRT_OPT_WAVES_PER_CU = 5, /* persistent waves per CU, 1..64; 0 (default): from the ray count */
 	PRT_OPT_LDS_STACK = 6,    /* LDS stack entries per lane: 8 (default) or 16 */
 	PRT_OPT_CHUNK = 7,        /* rays a wave reserves per atomic, 64..65536 (default 128) */
 	PRT_OPT_LEAF_PHASE = 8,   /* triangle phase when parked leaves >= live lanes * v / 64, 1..64 (default 20) */

@@ -489,7 +489,7 @@ int prt_mi355x_set_option(prt_ctx *c, int opt, int64_t v) {
 		c->tp.block = static_cast<int>(v);
 		break;
 	case PRT_OPT_WAVES_PER_CU:
-		if (v < 1 || v > 64) return bad();
+		if (v < 0 || v > 64) return bad();
 		c->tp.waves_per_cu = static_cast<int>(v);
 		break;
 	case PRT_OPT_LDS_STACK:

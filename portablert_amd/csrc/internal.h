@@ -125,7 +125,7 @@ struct TraceParams {
 	int node_format = 1;      // 0 fp32 nodes, 1 8-bit quantized nodes
 	int cull = 1;             // 0 exact (no distance culling), 1 distance culling with margin
 	int block = 128;          // threads per workgroup
-	int waves_per_cu = 32;    // persistent waves per CU
+	int waves_per_cu = 0;     // persistent waves per CU; 0: from the ray count (16..32)
 	int lds_stack = 8;        // LDS stack entries per thread (power of two)
 	int chunk = 128;          // rays a wave reserves per atomic (refilled lane by lane)
 	int leaf_phase = 20;      // triangle phase when parked leaves >= live lanes * leaf_phase / 64
