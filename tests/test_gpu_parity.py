@@ -144,6 +144,28 @@ def test_scheduling_options(gpu_ctx, opts):
     assert_same(got, want, fields=("t", "primitive_id", "u", "v"), what=str(opts))
 
 
+def test_default_wave_count(gpu_ctx):
+    """The default persistent-wave count (option 0) is chosen from the ray count: records equal
+    those of an explicit 32 waves per CU and the oracle's at sizes that pick 16, 22 and 32."""
+    tris = scenes.scene(2)
+    gpu_ctx.set_tris(tris)
+    assert gpu_ctx.get_option("waves_per_cu") == 0
+    mask = prt.T | prt.PRIMITIVE_ID
+    for count in (65536, 2073600, 4194304):
+        rays = scenes.rays(2, first=7, count=count)
+        print(f"default wave count, {count} rays", flush=True)
+        auto = gpu_ctx.nearest_hits(rays, mask)
+        try:
+            gpu_ctx.set_option("waves_per_cu", 32)
+            fixed = gpu_ctx.nearest_hits(rays, mask)
+        finally:
+            gpu_ctx.set_option("waves_per_cu", 0)
+        assert auto.tobytes() == fixed.tobytes()
+        idx = np.random.default_rng(count).choice(count, 20000, replace=False)
+        want = oracle.trace(tris, rays[idx], oracle.LOWEST_ID)
+        assert_same(auto[idx], want, fields=("t", "primitive_id"), what=f"{count} rays")
+
+
 def test_ray_order_does_not_matter(gpu_ctx):
     """All 8,388,608 room rays in a random order give the same record for every ray (a full-size
     property: which wave takes a ray and what runs beside it change nothing)."""
