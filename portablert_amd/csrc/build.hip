@@ -567,15 +567,32 @@ __global__ __launch_bounds__(kBlock) void k_scene_scale(const uint32_t *misc, ui
 	scale[s] = (a > 0.0 && a < 1e300) ? 1.0 / a : 1.0;
 }
 
+// Boxes (and collapse-cost rows) from the leaves to the roots: each leaf's thread climbs, and at
+// every internal node the second of its two children to arrive continues. A node whose leaf range
+// lies inside the workgroup's own leaves (most of them: a radix-tree node's range starts or ends
+// at its own index) is met by two threads of this workgroup, so its arrival counter and its
+// children's boxes and rows live in LDS; only nodes spanning two workgroups use the global
+// counters, which every XCD reaches through device-coherent atomics. Every node's box and row is
+// also written to global memory for the collapse.
 __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ tbox,
                                                      const uint32_t *__restrict__ sorted, BinTree t, CostCtx cc) {
-	const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+	// slots 0..kBlock-1: internal nodes b0 + s; kBlock..2 kBlock-1: leaves of sorted positions b0 + s
+	__shared__ uint32_t lflag[kBlock];
+	__shared__ float4 lbox[2 * kBlock][2];
+	__shared__ float lrow[kBlock][kCostRow];
+	const uint32_t b0 = blockIdx.x * kBlock;
+	const uint32_t b1 = min(b0 + kBlock, t.n) - 1;  // last leaf position of this workgroup
+	lflag[threadIdx.x] = 0u;
+	__syncthreads();
+	const uint32_t j = b0 + threadIdx.x;
 	if (j >= t.n) return;
 	uint32_t node = t.n - 1 + j;
 	const uint32_t tri = sorted[j];
 	float4 lo = tbox[2ull * tri], hi = tbox[2ull * tri + 1];
 	lo.w = 0.0f;  // hi.w: the never-cull flag, ORed up the tree with the boxes
 	st_box(t.box, node, lo, hi);
+	lbox[kBlock + threadIdx.x][0] = lo;
+	lbox[kBlock + threadIdx.x][1] = hi;
 	t.count[node] = 1;
 	t.first[node] = j;
 	float c[kCostRow];  // this node's collapse row (when costs are on)
@@ -583,11 +600,22 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
 	// a radix tree over at most 2^29 keys of at most 46 bits is at most 46 + 29 levels deep
 	for (int guard = 0; p != kNone && guard < 128; ++guard) {
-		stores_done();
-		if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
+		const uint32_t pf = t.first[p], pc = t.count[p];
+		const bool local = pf >= b0 && pf + pc - 1u <= b1;
 		const uint32_t sib = (t.left[p] == node) ? t.right[p] : t.left[p];
 		float4 sl, sh;
-		ld_box(t.box, sib, sl, sh);
+		if (local) {
+			// this thread's box (and row) are in LDS already; the release orders them before the
+			// count, the acquire the sibling's before the reads below
+			if (__hip_atomic_fetch_add(&lflag[p - b0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+			const uint32_t ss = sib >= t.n - 1 ? kBlock + (sib - (t.n - 1) - b0) : sib - b0;
+			sl = lbox[ss][0];
+			sh = lbox[ss][1];
+		} else {
+			stores_done();
+			if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
+			ld_box(t.box, sib, sl, sh);
+		}
 		lo.x = fminf(lo.x, sl.x);
 		lo.y = fminf(lo.y, sl.y);
 		lo.z = fminf(lo.z, sl.z);
@@ -599,15 +627,28 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 		if (cc.row) {
 			float cs[kCostRow], cp[kCostRow];
 			if (sib >= t.n - 1) leaf_row(rel_area(sl, sh, cc.scale[scene_of(t, cc, sib)]), cs);
-			else ld_row(cc.row, sib, cs);
+			else if (local) {
+#pragma unroll
+				for (int k = 0; k < kCostRow; ++k) cs[k] = lrow[sib - b0][k];
+			} else {
+				ld_row(cc.row, sib, cs);
+			}
 			const uint32_t scene = scene_of(t, cc, p);
 			int ls, w;
 			scene_shape(cc, scene, ls, w);
 			const bool left = t.left[p] == node;
-			node_row(rel_area(lo, hi, cc.scale[scene]), t.count[p], ls, w, left ? c : cs, left ? cs : c, cp);
+			node_row(rel_area(lo, hi, cc.scale[scene]), pc, ls, w, left ? c : cs, left ? cs : c, cp);
 #pragma unroll
 			for (int k = 0; k < kCostRow; ++k) c[k] = cp[k];
 			st_row(cc.row, p, c);
+			if (local) {
+#pragma unroll
+				for (int k = 0; k < kCostRow; ++k) lrow[p - b0][k] = c[k];
+			}
+		}
+		if (local) {
+			lbox[p - b0][0] = lo;
+			lbox[p - b0][1] = hi;
 		}
 		node = p;
 		p = t.parent[p];
