@@ -771,8 +771,9 @@ hipError_t plan_trace(const Bvh8 &b, TraceScratch &s, const TraceParams &p, uint
 	// at 32; 8 Mi room rays: 32 is fastest).
 	int wpc = p.waves_per_cu;
 	if (wpc <= 0) {
-		const uint64_t per_wave_count = static_cast<uint64_t>(s.n_cus) * 64u * 6u;
-		wpc = static_cast<int>(std::min<uint64_t>(32, std::max<uint64_t>(16, (n + per_wave_count - 1) / per_wave_count)));
+		const uint64_t rays_per_wave_per_cu = static_cast<uint64_t>(s.n_cus) * 64u * 6u;  // 6 rays a lane
+		wpc = static_cast<int>(
+		    std::min<uint64_t>(32, std::max<uint64_t>(16, (n + rays_per_wave_per_cu - 1) / rays_per_wave_per_cu)));
 	}
 	const uint64_t waves = static_cast<uint64_t>(s.n_cus) * static_cast<uint64_t>(wpc);
 	uint64_t grid64 = (waves * 64 + B - 1) / B;
