@@ -168,12 +168,13 @@ __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, 
 //
 // One workgroup (4 waves) takes 64 rays through the whole network: encoding, the three hidden
 // layers and the 4097 output logits, keeping the activations in LDS (X, 64 x 256 bf16) and
-// streaming W^T tiles of 64 output neurons (prefetched into registers one tile ahead) from L2.
-// In the hidden layers each wave computes a 32 x 32 part of every 64 x 64 tile with
-// v_mfma_f32_16x16x32_bf16 over K in steps of 32. For the output layer every wave loads all 64
-// rows of the last hidden layer into registers and owns 16 of each tile's 64 output neurons: it
-// fetches and stages only its own 16 rows of W^T (in its slice of two tile buffers, one of them
-// in X's space), and passes its 64 x 16 logits through the same slice to its 64 lanes, one ray
+// streaming W^T tiles of 64 output neurons (prefetched into registers one tile ahead) from L2,
+// with v_mfma_f32_16x16x32_bf16 over K in steps of 32. In every layer each wave loads all 64 rows
+// of the layer's input into registers and owns 16 of each tile's 64 neurons, so each W fragment
+// read from LDS serves 64 rows; a hidden layer writes its outputs to X as its tiles finish
+// (two barriers per layer). In the output layer each wave fetches and stages only its own 16
+// rows of W^T (in its slice of two tile buffers, one of them in X's space), and passes its
+// 64 x 16 logits through the same slice to its 64 lanes, one ray
 // each, which keep a running top-4 (value descending, index ascending). No wave reads another's
 // slice, so the 65 output tiles run without a barrier; the four waves' lists of a ray are merged
 // at the end, and no logit leaves the chip. The tiles are software-pipelined: a wave issues tile
@@ -193,80 +194,67 @@ struct FusedArgs {
 	uint32_t *top4;        // [rows][4]
 };
 
-// Global -> registers: the 64 x K bf16 tile of W^T rows [n0, n0 + 64), 16-byte chunks,
-// chunk c = tid + 256 q is row c / (K / 8), column group c % (K / 8).
+// One hidden layer computed per wave, without barriers between its tiles: every wave loads all 64
+// rows of X[:, :K] into registers, then owns 16 of each 64-neuron tile's columns and stages only
+// its own 16 rows of W^T in its slice of the W region (the next tile's rows prefetched into
+// registers during the MFMAs), so each W fragment read from LDS serves 64 rows. Once every wave
+// holds X, X's region takes the outputs as each tile finishes. Two barriers per layer; each
+// output element is the same MFMA chain over K in the same order as before.
 template <int K>
-__device__ __forceinline__ void tile_fetch(const uint16_t *wt, int n0, int tid, u32x4 pre[K / 32]) {
-	constexpr int per_row = K / 8;
+__device__ __forceinline__ void hidden_layer_pw(uint16_t *X, uint16_t *W, const uint16_t *wt, const float *bias,
+                                                int lane, int wave) {
+	constexpr int KS = K / kBK;       // MFMA K steps
+	constexpr int RC = K / 8;         // 16-byte chunks per W^T row
+	constexpr int CH = 16 * RC / 64;  // chunks per lane of a 16-row slice
+	bf16x8 xr[4][KS];
 #pragma unroll
-	for (int q = 0; q < K / 32; ++q) {
-		const int c = tid + 256 * q;
-		pre[q] = *reinterpret_cast<const u32x4 *>(wt + static_cast<size_t>(n0 + c / per_row) * K + (c % per_row) * 8);
-	}
-}
-
-template <int K>
-__device__ __forceinline__ void tile_store(uint16_t *W, int tid, const u32x4 pre[K / 32]) {
-	constexpr int per_row = K / 8;
+	for (int i = 0; i < 4; ++i)
 #pragma unroll
-	for (int q = 0; q < K / 32; ++q) {
-		const int c = tid + 256 * q;
-		*reinterpret_cast<u32x4 *>(W + (c / per_row) * kFStride + (c % per_row) * 8) = pre[q];
-	}
-}
-
-// acc (this wave's 32 x 32 part at rows wr, columns wc) = X[:, :K] * W[:, :K]^T
-template <int K>
-__device__ __forceinline__ void tile_mma(const uint16_t *X, const uint16_t *W, int wr, int wc, int lane,
-                                         f32x4 acc[2][2]) {
+		for (int kk = 0; kk < KS; ++kk)
+			xr[i][kk] = *reinterpret_cast<const bf16x8 *>(X + (i * 16 + (lane & 15)) * kFStride + kk * kBK + (lane >> 4) * 8);
+	uint16_t *ws = W + wave * 16 * kFStride;  // this wave's slice
+	auto fetch = [&](int t, u32x4 q[CH]) {
 #pragma unroll
-	for (int i = 0; i < 2; ++i)
-#pragma unroll
-		for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-	for (int k0 = 0; k0 < K; k0 += kBK) {
-		bf16x8 af[2], bfr[2];
-#pragma unroll
-		for (int s = 0; s < 2; ++s) {
-			af[s] = *reinterpret_cast<const bf16x8 *>(X + (wr + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
-			bfr[s] = *reinterpret_cast<const bf16x8 *>(W + (wc + s * 16 + (lane & 15)) * kFStride + k0 + (lane >> 4) * 8);
+		for (int j = 0; j < CH; ++j) {
+			const int c = lane + 64 * j;
+			q[j] = *reinterpret_cast<const u32x4 *>(wt + static_cast<size_t>(t * kFCols + wave * 16 + c / RC) * K + (c % RC) * 8);
 		}
+	};
+	auto store = [&](const u32x4 q[CH]) {
 #pragma unroll
-		for (int i = 0; i < 2; ++i)
-#pragma unroll
-			for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-	}
-}
-
-// One hidden layer over the 64 rays of X (input width K, 256 outputs in four tiles of 64; the
-// four accumulator sets stay in registers until X can be overwritten).
-template <int K>
-__device__ __forceinline__ void hidden_layer(uint16_t *X, uint16_t *W, const uint16_t *wt, const float *bias, int tid,
-                                             int wr, int wc, int lane) {
-	f32x4 acc[4][2][2];
+		for (int j = 0; j < CH; ++j) {
+			const int c = lane + 64 * j;
+			*reinterpret_cast<u32x4 *>(ws + (c / RC) * kFStride + (c % RC) * 8) = q[j];
+		}
+	};
+	u32x4 pre[CH];
+	fetch(0, pre);
+	__syncthreads();  // every wave holds X in registers: X's region takes the outputs
+	store(pre);
 #pragma unroll
 	for (int t = 0; t < 4; ++t) {
-		{
-			u32x4 pre[K / 32];
-			tile_fetch<K>(wt, t * kFCols, tid, pre);
-			tile_store<K>(W, tid, pre);
+		if (t + 1 < 4) fetch(t + 1, pre);
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		f32x4 acc[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+		for (int kk = 0; kk < KS; ++kk) {
+			const bf16x8 bfr = *reinterpret_cast<const bf16x8 *>(ws + (lane & 15) * kFStride + kk * kBK + (lane >> 4) * 8);
+#pragma unroll
+			for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[i][kk], bfr, acc[i], 0, 0, 0);
 		}
-		__syncthreads();
-		tile_mma<K>(X, W, wr, wc, lane, acc[t]);
-		__syncthreads();
+		if (t + 1 < 4) store(pre);  // this wave's MFMAs have read the slice (LDS operations of one wave complete in order)
+		const int col = t * kFCols + wave * 16 + (lane & 15);
+		const float bj = bias[col];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int e = 0; e < 4; ++e)
+				X[(i * 16 + (lane >> 4) * 4 + e) * kFStride + col] = f2bf(fmaxf(acc[i][e] + bj, 0.0f));
 	}
-#pragma unroll
-	for (int t = 0; t < 4; ++t)
-#pragma unroll
-		for (int j = 0; j < 2; ++j) {
-			const int col = t * kFCols + wc + j * 16 + (lane & 15);
-			const float bj = bias[col];
-#pragma unroll
-			for (int i = 0; i < 2; ++i)
-#pragma unroll
-				for (int e = 0; e < 4; ++e)
-					X[(wr + i * 16 + (lane >> 4) * 4 + e) * kFStride + col] = f2bf(fmaxf(acc[t][i][j][e] + bj, 0.0f));
-		}
 	__syncthreads();
 }
 
@@ -305,7 +293,6 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	__shared__ __align__(16) uint16_t X[kFRows * kFStride];
 	__shared__ __align__(16) uint16_t W[kFCols * kFStride];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 	const int r0 = blockIdx.x * kFRows;
 	// encoding: two threads per ray, three inputs each; rows past the end are zero
 	if (tid < 2 * kFRows) {
@@ -318,9 +305,9 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 		}
 	}
 	__syncthreads();
-	hidden_layer<kMlpIn>(X, W, a.w[0], a.b[0], tid, wr, wc, lane);
-	hidden_layer<kMlpHidden>(X, W, a.w[1], a.b[1], tid, wr, wc, lane);
-	hidden_layer<kMlpHidden>(X, W, a.w[2], a.b[2], tid, wr, wc, lane);
+	hidden_layer_pw<kMlpIn>(X, W, a.w[0], a.b[0], lane, wave);
+	hidden_layer_pw<kMlpHidden>(X, W, a.w[1], a.b[1], lane, wave);
+	hidden_layer_pw<kMlpHidden>(X, W, a.w[2], a.b[2], lane, wave);
 	// output layer: all 64 rows of the last hidden layer in every wave's registers; each wave
 	// computes 16 of a tile's 64 columns, so each W fragment read from LDS serves 64 rows
 	const int oc = wave * 16;
