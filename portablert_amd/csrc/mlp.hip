@@ -154,6 +154,32 @@ __device__ __forceinline__ void encode_inputs(const float *r, float4 lo, float4 
 	}
 }
 
+// Inputs a0..a0+2, frequencies j0..j0+7 of one ray, each (sin, cos) pair as one 32-bit store:
+// the fused kernel's four threads per ray (the same values as encode_inputs).
+__device__ __forceinline__ void encode_quarter(const float *r, float4 lo, float4 inv_ext, int a0, int j0, uint16_t *o) {
+	float in[3];
+	if (a0 == 0) {
+		in[0] = (r[0] - lo.x) * inv_ext.x * 2.0f - 1.0f;
+		in[1] = (r[1] - lo.y) * inv_ext.y * 2.0f - 1.0f;
+		in[2] = (r[2] - lo.z) * inv_ext.z * 2.0f - 1.0f;
+	} else {
+		in[0] = r[3];
+		in[1] = r[4];
+		in[2] = r[5];
+	}
+	uint32_t *o32 = reinterpret_cast<uint32_t *>(o);
+#pragma unroll
+	for (int q = 0; q < 3; ++q) {
+#pragma unroll
+		for (int jj = 0; jj < 8; ++jj) {
+			const int j = j0 + jj;
+			const float y = in[q] * (0.5f * exp2f(0.5f * static_cast<float>(j)));
+			o32[(a0 + q) * 16 + j] = static_cast<uint32_t>(f2bf(__builtin_amdgcn_sinf(y))) |
+			                         (static_cast<uint32_t>(f2bf(__builtin_amdgcn_cosf(y))) << 16);
+		}
+	}
+}
+
 __global__ __launch_bounds__(256) void k_encode(const float *__restrict__ rays, uint64_t first, uint32_t n, float4 lo,
                                                 float4 inv_ext, uint16_t *__restrict__ x) {
 	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -295,13 +321,18 @@ __global__ __launch_bounds__(256, 2) void k_mlp_top4(FusedArgs a) {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int r0 = blockIdx.x * kFRows;
 	// encoding: two threads per ray, three inputs each; rows past the end are zero
-	if (tid < 2 * kFRows) {
-		const int row = tid >> 1, half = tid & 1;
+	{
+		// four threads per ray: inputs 0-2 or 3-5, frequencies 0-7 or 8-15
+		const int row = tid >> 2, half = (tid >> 1) & 1, fq = tid & 1;
 		uint16_t *o = X + row * kFStride;
 		if (r0 + row < a.rows) {
-			encode_inputs(a.rays + 6 * (a.first + r0 + row), a.lo, a.inv_ext, half * 3, o);
+			encode_quarter(a.rays + 6 * (a.first + r0 + row), a.lo, a.inv_ext, half * 3, fq * 8, o);
 		} else {
-			for (int c = half * 96; c < half * 96 + 96; ++c) o[c] = 0;
+			uint32_t *o32 = reinterpret_cast<uint32_t *>(o);
+#pragma unroll
+			for (int q = 0; q < 3; ++q)
+#pragma unroll
+				for (int jj = 0; jj < 8; ++jj) o32[(half * 3 + q) * 16 + fq * 8 + jj] = 0u;
 		}
 	}
 	__syncthreads();
