@@ -881,6 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_level_advance(uint32_t *misc) {
 // Collapses BVH8 node i (binary node bin_of[i]): picks its children, allocates its internal
 // children (their binary nodes go to bin_of) and its leaves' triangle slots, and writes both
 // node formats. Returns {first internal child, number of internal children}.
+template <bool COST>
 __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i) {
 	const BinTree &t = a.t;
 	const uint32_t n_int = t.n - 1;  // binary internal nodes are ids < n_int
@@ -902,7 +903,7 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 	scene_shape(a.cc, b != kNone ? scene_of(t, a.cc, b) : 0u, leaf_size, width);
 	// is_leaf bit k: child k becomes a leaf (by the cost rows, or by its size when greedy)
 	uint32_t is_leaf = 0;
-	if (a.cc.row && ne == 2) {
+	if (COST && ne == 2) {
 		// the width slots split between the two subtrees where that costs least, and each
 		// part again wherever using fewer slots does not cost as little (at most 8 parts)
 		float rl[kCostRow], rr[kCostRow], cx[kCostRow];
@@ -948,7 +949,7 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 	for (int k = 0; k < 8; ++k) ar[k] = -1.0f;
 	for (int k = 0; k < ne; ++k)
 		if (el[k] < n_int) ar[k] = half_area(t.box[2ull * el[k]], t.box[2ull * el[k] + 1]);
-	while (!a.cc.row && ne < width) {
+	while (!COST && ne < width) {
 		int best = -1;
 		float ba = -1.0f;
 #pragma unroll
@@ -972,7 +973,7 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 	uint32_t n_leaf_tris = 0;
 	for (int k = 0; k < ne; ++k) {
 		cnt[k] = t.count[el[k]];
-		if (!a.cc.row && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
+		if (!COST && cnt[k] <= static_cast<uint32_t>(leaf_size)) is_leaf |= 1u << k;
 		if (!((is_leaf >> k) & 1u)) ++n_inner;
 		else n_leaf_tris += cnt[k];
 	}
@@ -1060,9 +1061,10 @@ __device__ __forceinline__ uint2 collapse_node(const CollapseArgs &a, uint32_t i
 	return make_uint2(child_base, static_cast<uint32_t>(n_inner));
 }
 
+template <bool COST>
 __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
-	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) collapse_node(a, i);
+	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) collapse_node<COST>(a, i);
 }
 
 // The rest of the tree below the current level, depth first, one frontier node per thread: a
@@ -1071,6 +1073,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(CollapseArgs a) {
 // one {first node, nodes left} range per level; a deeper tree sets error bit 2.
 constexpr int kDfsStack = kFixedDfsDepth;
 
+template <bool COST>
 __global__ __launch_bounds__(kBlock) void k_collapse_dfs(CollapseArgs a) {
 	const uint32_t begin = a.misc[M_LVL_BEGIN], end = a.misc[M_LVL_END];
 	for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end; i += gridDim.x * kBlock) {
@@ -1081,7 +1084,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse_dfs(CollapseArgs a) {
 			const uint32_t node = st[sp - 1].x;
 			if (st[sp - 1].y > 1u) st[sp - 1] = make_uint2(node + 1u, st[sp - 1].y - 1u);
 			else --sp;
-			const uint2 ch = collapse_node(a, node);
+			const uint2 ch = collapse_node<COST>(a, node);
 			if (ch.y == 0u) continue;
 			if (sp == kDfsStack) {
 				atomicOr(a.misc + M_ERR, 4u);
@@ -1419,7 +1422,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
 	for (int level = 0; level < max_levels; ++level) {
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
-		hipLaunchKernelGGL(k_collapse, dim3(grid), dim3(kBlock), 0, st, ca);
+		if (ca.cc.row) hipLaunchKernelGGL(k_collapse<true>, dim3(grid), dim3(kBlock), 0, st, ca);
+		else hipLaunchKernelGGL(k_collapse<false>, dim3(grid), dim3(kBlock), 0, st, ca);
 		if (!fixed && (level & 3) == 3) {
 			PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
 			PRT_TRY(hipStreamSynchronize(st));
@@ -1429,7 +1433,8 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	}
 	if (fixed) {
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
-		hipLaunchKernelGGL(k_collapse_dfs, dim3(grid), dim3(kBlock), 0, st, ca);
+		if (ca.cc.row) hipLaunchKernelGGL(k_collapse_dfs<true>, dim3(grid), dim3(kBlock), 0, st, ca);
+		else hipLaunchKernelGGL(k_collapse_dfs<false>, dim3(grid), dim3(kBlock), 0, st, ca);
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);  // every node is done
 	}
 	hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(n)), dim3(kBlock), 0, st, tri_in, s.vals[cur].as<uint32_t>(), keys,
