@@ -598,11 +598,33 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	float c[kCostRow];  // this node's collapse row (when costs are on)
 	if (cc.row) leaf_row(rel_area(lo, hi, cc.scale[scene_of(t, cc, node)]), c);
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
+	// the parent's links (range, children, its own parent); no thread of this kernel writes them,
+	// so the next level's are fetched before this level's arrival counter and the climb waits
+	// on the counters and the sibling boxes only
+	uint32_t pf = 0, pc = 0, pl = 0, pr = 0, pp = kNone;
+	if (p != kNone) {
+		pf = t.first[p];
+		pc = t.count[p];
+		pl = t.left[p];
+		pr = t.right[p];
+		pp = t.parent[p];
+	}
 	// a radix tree over at most 2^29 keys of at most 46 bits is at most 46 + 29 levels deep
 	for (int guard = 0; p != kNone && guard < 128; ++guard) {
-		const uint32_t pf = t.first[p], pc = t.count[p];
 		const bool local = pf >= b0 && pf + pc - 1u <= b1;
-		const uint32_t sib = (t.left[p] == node) ? t.right[p] : t.left[p];
+		const bool left = pl == node;
+		const uint32_t sib = left ? pr : pl;
+		// this thread's box (and row) stores complete before the device-scope count (the wait
+		// is for every load too, so it comes before the next level's links are fetched)
+		if (!local) stores_done();
+		uint32_t qf = 0, qc = 0, ql = 0, qr = 0, qp = kNone;
+		if (pp != kNone) {
+			qf = t.first[pp];
+			qc = t.count[pp];
+			ql = t.left[pp];
+			qr = t.right[pp];
+			qp = t.parent[pp];
+		}
 		float4 sl, sh;
 		if (local) {
 			// this thread's box (and row) are in LDS already; the release orders them before the
@@ -612,7 +634,6 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 			sl = lbox[ss][0];
 			sh = lbox[ss][1];
 		} else {
-			stores_done();
 			if (atomicAdd(&t.flag[p], 1u) == 0u) return;  // sibling not finished: it continues
 			ld_box(t.box, sib, sl, sh);
 		}
@@ -636,7 +657,6 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 			const uint32_t scene = scene_of(t, cc, p);
 			int ls, w;
 			scene_shape(cc, scene, ls, w);
-			const bool left = t.left[p] == node;
 			node_row(rel_area(lo, hi, cc.scale[scene]), pc, ls, w, left ? c : cs, left ? cs : c, cp);
 #pragma unroll
 			for (int k = 0; k < kCostRow; ++k) c[k] = cp[k];
@@ -651,7 +671,12 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 			lbox[p - b0][1] = hi;
 		}
 		node = p;
-		p = t.parent[p];
+		p = pp;
+		pf = qf;
+		pc = qc;
+		pl = ql;
+		pr = qr;
+		pp = qp;
 	}
 }
 
