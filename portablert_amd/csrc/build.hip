@@ -573,13 +573,15 @@ __global__ __launch_bounds__(kBlock) void k_scene_scale(const uint32_t *misc, ui
 // at its own index) is met by two threads of this workgroup, so its arrival counter and its
 // children's boxes and rows live in LDS; only nodes spanning two workgroups use the global
 // counters, which every XCD reaches through device-coherent atomics. Every node's box and row is
-// also written to global memory for the collapse.
+// also written to global memory for the collapse. Compiled once per cost mode: without the rows
+// a workgroup holds 17 KB of LDS instead of 25 KB, so eight fit on a CU instead of six.
+template <bool COST>
 __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ tbox,
                                                      const uint32_t *__restrict__ sorted, BinTree t, CostCtx cc) {
 	// slots 0..kBlock-1: internal nodes b0 + s; kBlock..2 kBlock-1: leaves of sorted positions b0 + s
 	__shared__ uint32_t lflag[kBlock];
 	__shared__ float4 lbox[2 * kBlock][2];
-	__shared__ float lrow[kBlock][kCostRow];
+	__shared__ float lrow[COST ? kBlock : 1][kCostRow];
 	const uint32_t b0 = blockIdx.x * kBlock;
 	const uint32_t b1 = min(b0 + kBlock, t.n) - 1;  // last leaf position of this workgroup
 	lflag[threadIdx.x] = 0u;
@@ -596,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 	t.count[node] = 1;
 	t.first[node] = j;
 	float c[kCostRow];  // this node's collapse row (when costs are on)
-	if (cc.row) leaf_row(rel_area(lo, hi, cc.scale[scene_of(t, cc, node)]), c);
+	if (COST) leaf_row(rel_area(lo, hi, cc.scale[scene_of(t, cc, node)]), c);
 	uint32_t p = (t.n > 1) ? t.parent[node] : kNone;
 	// the parent's links (range, children, its own parent); no thread of this kernel writes them,
 	// so the next level's are fetched before this level's arrival counter and the climb waits
@@ -645,7 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_bottomup(const float4 *__restrict__ 
 		hi.z = fmaxf(hi.z, sh.z);
 		hi.w = __uint_as_float(__float_as_uint(hi.w) | __float_as_uint(sh.w));
 		st_box(t.box, p, lo, hi);
-		if (cc.row) {
+		if (COST) {
 			float cs[kCostRow], cp[kCostRow];
 			if (sib >= t.n - 1) leaf_row(rel_area(sl, sh, cc.scale[scene_of(t, cc, sib)]), cs);
 			else if (local) {
@@ -1429,8 +1431,12 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 		hipLaunchKernelGGL(k_scene_init, dim3(blocks_for(S)), dim3(kBlock), 0, st, off, S, n, scene_root);
 		if (n > 1)
 			hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kBlock), 0, st, keys, t, scene_root);
-		hipLaunchKernelGGL(k_bottomup, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
-		                   s.vals[cur].as<uint32_t>(), t, cc);
+		if (cc.row)
+			hipLaunchKernelGGL(k_bottomup<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
+			                   s.vals[cur].as<uint32_t>(), t, cc);
+		else
+			hipLaunchKernelGGL(k_bottomup<false>, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tbox.as<float4>(),
+			                   s.vals[cur].as<uint32_t>(), t, cc);
 	}
 	hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, s.tri_dest.as<uint32_t>(), n, kNone);
 	hipLaunchKernelGGL(k_roots, dim3(blocks_for(S)), dim3(kBlock), 0, st, misc, s.bin_of.as<uint32_t>(), scene_root, S);
