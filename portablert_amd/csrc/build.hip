@@ -293,32 +293,59 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t *hist, uint32_t total) {
 	}
 }
 
-// Stable scatter: keys of tile t are ranked in index order (round-major, then wave, then
-// lane); equal digits are matched within a wave with eight ballots.
+// Stable scatter through LDS. A tile's keys are loaded at once (kSortItems per thread), ranked
+// in index order (round-major, then wave, then lane; equal digits matched within a wave with
+// eight ballots) and placed in LDS in digit order; the tile is then written out so that
+// consecutive threads write consecutive positions of a digit's run. The tile's count of a digit
+// is the difference of two consecutive entries of the scanned histogram: it is digit-major, so
+// the entry after a digit's last tile is the next digit's first.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                     uint32_t n, int shift, const uint32_t *__restrict__ hist,
                                                     const uint32_t *__restrict__ seg, uint32_t tiles) {
-	__shared__ uint32_t offs[256];
-	__shared__ uint32_t wcnt[kBlock / 64][256];
+	__shared__ uint64_t skey[kSortTile];
+	__shared__ uint32_t sval[kSortTile];
+	__shared__ uint32_t run[256];                // next tile position of each digit
+	__shared__ uint32_t delta[256];              // output position minus tile position, per digit
+	__shared__ uint8_t wcnt[kBlock / 64][256];  // keys of each digit in each wave of a round (<= 64)
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const uint64_t h = static_cast<uint64_t>(threadIdx.x) * tiles + blockIdx.x;
-	offs[threadIdx.x] = hist[h] + seg[h / kScanSeg];
+	const uint32_t d = threadIdx.x;
+	const uint64_t total = 256ull * tiles;
+	const uint64_t e = static_cast<uint64_t>(d) * tiles + blockIdx.x;
+	const uint32_t g0 = hist[e] + seg[e / kScanSeg];
+	const uint32_t g1 = e + 1 < total ? hist[e + 1] + seg[(e + 1) / kScanSeg] : n;
+	const uint32_t cnt = g1 - g0;
 	const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+	uint64_t kk[kSortItems];
+	uint32_t vv[kSortItems];
+#pragma unroll
+	for (int r = 0; r < kSortItems; ++r) {
+		const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
+		kk[r] = i < n ? kin[i] : 0ull;
+		vv[r] = i < n ? vin[i] : 0u;
+	}
+	// exclusive scan of the tile's digit counts: each digit's first tile position
+	run[d] = cnt;
+	__syncthreads();
+	for (int sh = 1; sh < 256; sh <<= 1) {
+		const uint32_t a = d >= static_cast<uint32_t>(sh) ? run[d - sh] : 0u;
+		__syncthreads();
+		run[d] += a;
+		__syncthreads();
+	}
+	const uint32_t lofs = run[d] - cnt;
+	run[d] = lofs;
+	delta[d] = g0 - lofs;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
 	for (int r = 0; r < kSortItems; ++r) {
 #pragma unroll
 		for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
 		__syncthreads();
 		const uint64_t i = base + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
 		const bool act = i < n;
-		uint64_t k = 0;
-		uint32_t v = 0, digit = 0;
-		if (act) {
-			k = kin[i];
-			v = vin[i];
-			digit = static_cast<uint32_t>(k >> shift) & 0xFFu;
-		}
+		const uint64_t k = kk[r];
+		const uint32_t digit = static_cast<uint32_t>(k >> shift) & 0xFFu;
 		uint64_t peers = __ballot(act);
 #pragma unroll
 		for (int b = 0; b < 8; ++b) {
@@ -327,23 +354,27 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint64_t *__restrict__
 			peers &= bit ? m : ~m;
 		}
 		const uint32_t rank = __popcll(peers & lt);
-		if (act && rank == 0) wcnt[wave][digit] = __popcll(peers);
+		if (act && rank == 0) wcnt[wave][digit] = static_cast<uint8_t>(__popcll(peers));
 		__syncthreads();
-		uint32_t pos = 0;
 		if (act) {
-			pos = offs[digit] + rank;
+			uint32_t pos = run[digit] + rank;
 			for (int w = 0; w < wave; ++w) pos += wcnt[w][digit];
+			skey[pos] = k;
+			sval[pos] = vv[r];
 		}
 		__syncthreads();
 		uint32_t add = 0;
 #pragma unroll
 		for (int w = 0; w < kBlock / 64; ++w) add += wcnt[w][threadIdx.x];
-		offs[threadIdx.x] += add;
-		if (act) {
-			kout[pos] = k;
-			vout[pos] = v;
-		}
-		__syncthreads();
+		run[threadIdx.x] += add;
+	}
+	__syncthreads();
+	const uint32_t m = static_cast<uint32_t>(min(static_cast<uint64_t>(kSortTile), n - base));
+	for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+		const uint64_t k = skey[j];
+		const uint32_t g = delta[static_cast<uint32_t>(k >> shift) & 0xFFu] + j;
+		kout[g] = k;
+		vout[g] = sval[j];
 	}
 }
 
