@@ -1482,10 +1482,15 @@ hipError_t build_bvh8(BuildScratch &s, uint64_t n64, const BuildParams &p, Bvh8 
 	// a fixed-length build: a few level launches, then the rest of the tree depth first
 	const int max_levels = fixed ? std::min(ss.levels, kFixedTopLevels) : 128;
 	const uint32_t grid = std::min<uint32_t>(blocks_for(cap), 2048);
+	uint64_t level_nodes = S;  // most BVH8 nodes a level can hold: the roots, then 8 children each
 	for (int level = 0; level < max_levels; ++level) {
+		// a grid for that many nodes (the kernel's loop covers any count), so that the small top
+		// levels do not dispatch thousands of idle waves
+		const uint32_t lgrid = std::min<uint32_t>(grid, blocks_for(level_nodes));
+		level_nodes = std::min<uint64_t>(level_nodes * 8u, cap);
 		hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st, misc);
-		if (ca.cc.row) hipLaunchKernelGGL(k_collapse<true>, dim3(grid), dim3(kBlock), 0, st, ca);
-		else hipLaunchKernelGGL(k_collapse<false>, dim3(grid), dim3(kBlock), 0, st, ca);
+		if (ca.cc.row) hipLaunchKernelGGL(k_collapse<true>, dim3(lgrid), dim3(kBlock), 0, st, ca);
+		else hipLaunchKernelGGL(k_collapse<false>, dim3(lgrid), dim3(kBlock), 0, st, ca);
 		if (!fixed && (level & 3) == 3) {
 			PRT_TRY(hipMemcpyAsync(hm, misc, sizeof(uint32_t) * misc_words(1), hipMemcpyDeviceToHost, st));
 			PRT_TRY(hipStreamSynchronize(st));
